@@ -1,0 +1,52 @@
+// Batched GPU engine: packs many independent Fokker-Planck solves into flat
+// device arrays and runs the assemble -> eigen -> ADMM -> Radau -> payoff
+// kernels over all of them.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "domain.hpp"
+
+namespace fpe {
+
+struct SolveResult {
+    int status = 0, n_acc = 0, n_rej = 0, n_lu = 0, n_newt = 0, n_err = 0, n_inner = 0, n_admm = 0;
+    int nsb = 0, nvb = 0;
+    std::vector<double> q0, qT;              // filled when requested
+    std::vector<std::vector<double>> hist;   // accepted states (stepwise API)
+    std::vector<double> pdf;                 // n_qs x n_qv (stepwise API)
+    std::vector<double> bas[2];              // per-point basis tables (8 doubles) for s, v
+    std::vector<int> bas_c0[2];              // first recombined column per point
+};
+
+struct BatchTimes {
+    double setup = 0, assemble = 0, ic = 0, radau = 0, payoff = 0, total = 0;
+};
+
+struct Request {
+    int solve;         // index into the solve list
+    int option_type;   // payoff type of that solve
+    double barrier, strike, sign;
+    int out;           // output slot
+};
+
+struct RunOptions {
+    bool want_q0 = false, want_qT = false, want_pdf = false, want_basis = false;
+    int hist_cap = 0;  // > 0: keep up to this many accepted states per solve
+    bool stop_after_ic = false;
+};
+
+// Runs solves + pricing requests; out_prices has n_out entries (accumulated).
+// Returns false with a message in err on failure.
+bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& domains,
+               const std::vector<Request>& reqs, int n_out, const RunOptions& opt, std::vector<double>& out_prices,
+               std::vector<SolveResult>& results, BatchTimes& times, std::string& err);
+
+// Price options (with in-out parity); status per option.
+bool price_options(const std::vector<OptionSpec>& opts, std::vector<double>& prices, std::vector<int>& status,
+                   std::vector<SolveResult>* results, BatchTimes* times, std::string& err);
+
+int device_count();
+
+}  // namespace fpe

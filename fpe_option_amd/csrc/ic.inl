@@ -1,0 +1,303 @@
+// Initial condition: Galerkin projection of a truncated Gaussian at (S0, V0)
+// followed by the reference's ADMM non-negative least-squares fit
+// (src/engines/fpe/initial_cond.mojo:25-131, src/numerics/optim/osqp.mojo:52-159).
+//
+// The ADMM x-update solves (A^T A + alpha I) x = rhs with A = D M D, where
+// M = Ns (x) Nv and D = diag(M)^(-1/2) = Ds (x) Dv, so A = As (x) Av with
+// As = Ds Ns Ds, Av = Dv Nv Dv symmetric. With As = Qs Ls Qs^T and
+// Av = Qv Lv Qv^T the solve is a fast diagonalisation:
+//     x = (Qs (x) Qv) [ (Qs^T R Qv) ./ (ls^2 lv^2 + alpha) ] (Qs (x) Qv)^T,
+// i.e. four small dense products per iteration instead of a banded
+// factor of bandwidth 2(3 n_vb + 3) streamed from memory every iteration.
+#pragma once
+#include "device_common.hpp"
+
+namespace fpe {
+
+// EG layout per instance: Qs (ns^2), Qv (nv^2), QvT (nv^2), ls (ns), lv (nv), As (ns^2), Av (nv^2)
+struct EigView {
+    double *Qs, *Qv, *QvT, *ls, *lv, *As, *Av;
+    __device__ EigView(double* base, int ns, int nv) {
+        Qs = base;
+        Qv = Qs + (long long)ns * ns;
+        QvT = Qv + (long long)nv * nv;
+        ls = QvT + (long long)nv * nv;
+        lv = ls + ns;
+        As = lv + nv;
+        Av = As + (long long)ns * ns;
+    }
+};
+
+// Right-hand side b~ = D (Bs (x) Bv)^T W delta and the basis integrals
+// m = (Bs (x) Bv)^T W 1 (separable).  grid: n_inst.
+__global__ void __launch_bounds__(NT) k_ic_rhs(const Inst* __restrict__ inst, const double* __restrict__ QD,
+                                               const double* __restrict__ BV, const int* __restrict__ BC,
+                                               double* __restrict__ VC) {
+    const Inst& in = inst[blockIdx.x];
+    const int ns = in.nsb, nv = in.nvb, N = ns * nv, nqs = in.nqs, nqv = in.nqv;
+    const double* sw = QD + in.o_sq + nqs;
+    const double* sph = QD + in.o_sq + 2 * nqs;
+    const double* vx = QD + in.o_vq;
+    const double* vw = vx + nqv;
+    const double* Ts = BV + in.o_bs;
+    const double* Tv = BV + in.o_bv;
+    const int* cs = BC + in.o_bs / 8;
+    const int* cv = BC + in.o_bv / 8;
+    double* b = VC + in.o_vec + (long long)V_BT * N;
+    double* mw = VC + in.o_vec + (long long)V_MW * N;
+    const double* Di = VC + in.o_vec + (long long)V_DINV * N;
+    __shared__ double om_s[320], om_v[320];
+    for (int r = threadIdx.x; r < N; r += NT) b[r] = 0.0;
+    // omega_s[k] = sum_p Bs[p,k] sw_p ; omega_v[l] = sum_q Bv[q,l] vw_q
+    for (int k = threadIdx.x; k < ns; k += NT) {
+        double a = 0.0;
+        const int p0 = lower_c0(cs, nqs, k - 3), p1 = lower_c0(cs, nqs, k + 1);
+        for (int p = p0; p < p1; ++p) a += Ts[(long long)p * 8 + (k - cs[p])] * sw[p];
+        om_s[k] = a;
+    }
+    for (int l = threadIdx.x; l < nv; l += NT) {
+        double a = 0.0;
+        const int q0 = lower_c0(cv, nqv, l - 3), q1 = lower_c0(cv, nqv, l + 1);
+        for (int q = q0; q < q1; ++q) a += Tv[(long long)q * 8 + (l - cv[q])] * vw[q];
+        om_v[l] = a;
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < N; r += NT) mw[r] = om_s[r / nv] * om_v[r % nv];
+    // truncated product Gaussian (rho = 0), initial_cond.mojo:25-67
+    const double sig_s = in.ic_sig_s, sig_v = in.ic_sig_v;
+    const double norm = 1.0 / (2.0 * M_PI * sig_s * sig_v * 1.0);
+    __syncthreads();
+    for (int p = 0; p < nqs; ++p) {
+        const double ds = (sph[p] - in.S0) / sig_s, ds2 = ds * ds;
+        if (ds2 > 50.0) continue;  // uniform across the workgroup
+        for (int l = threadIdx.x; l < nv; l += NT) {
+            const int q0 = lower_c0(cv, nqv, l - 3), q1 = lower_c0(cv, nqv, l + 1);
+            double T = 0.0;
+            for (int q = q0; q < q1; ++q) {
+                const double dv = (vx[q] - in.V0) / sig_v;
+                const double z = ds2 - 2.0 * 0.0 * ds * dv + dv * dv;
+                double dl = 0.0;
+                if (!(z * 0.5 > 50.0)) {
+                    dl = norm * exp(-z * 0.5);
+                    if (dl < 1e-6) dl = 0.0;
+                }
+                T += Tv[(long long)q * 8 + (l - cv[q])] * (sw[p] * vw[q] * dl);
+            }
+            for (int t = 0; t < 4; ++t) {
+                const int k = cs[p] + t;
+                if (k < ns) b[(long long)k * nv + l] += Ts[(long long)p * 8 + t] * T;
+            }
+        }
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < N; r += NT) b[r] = Di[r] * b[r];
+}
+
+// Jacobi eigendecomposition of As (axis 0) or Av (axis 1) with parallel
+// round-robin ordering.  grid: (n_inst, 2).
+__global__ void __launch_bounds__(NT) k_eig(const Inst* __restrict__ inst, const double* __restrict__ FC,
+                                            double* __restrict__ EG) {
+    const Inst& in = inst[blockIdx.x];
+    const int ax = blockIdx.y;
+    const int ns = in.nsb, nv = in.nvb;
+    EigView E(EG + in.o_eig, ns, nv);
+    const int n = ax == 0 ? ns : nv;
+    const double* F = FC + (ax == 0 ? in.o_fs : in.o_fv);  // factor 0 = mass
+    double* A = ax == 0 ? E.As : E.Av;
+    double* Q = ax == 0 ? E.Qs : E.Qv;
+    double* lam = ax == 0 ? E.ls : E.lv;
+    __shared__ double cc[160], ss[160], red[NWAVE];
+    __shared__ int pp[160], qq[160];
+    for (int e = threadIdx.x; e < n * n; e += NT) {
+        const int i = e / n, j = e - i * n;
+        double v = 0.0;
+        if (j - i >= -3 && j - i <= 3) {
+            const double mii = F[(long long)i * 7 + 3], mjj = F[(long long)j * 7 + 3];
+            v = F[(long long)i * 7 + (j - i + 3)] / sqrt(mii * mjj);
+        }
+        A[e] = v;
+        Q[e] = (i == j) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    const int np = n + (n & 1);
+    const int npair = np / 2;
+    for (int sweep = 0; sweep < 30; ++sweep) {
+        double off = 0.0, dg = 0.0;
+        for (int e = threadIdx.x; e < n * n; e += NT) {
+            const int i = e / n, j = e - i * n;
+            const double v = A[e] * A[e];
+            if (i == j) dg += v; else off += v;
+        }
+        double v2[2] = {off, dg};
+        __shared__ double red2[2 * NWAVE];
+        block_sum_n<2>(v2, red2);
+        if (v2[0] <= 1e-32 * v2[1]) break;
+        for (int step = 0; step < np - 1; ++step) {
+            for (int k = threadIdx.x; k < npair; k += NT) {
+                auto idx = [&](int a) { return a == 0 ? 0 : 1 + (a - 1 + step) % (np - 1); };
+                int p = idx(k), q = idx(np - 1 - k);
+                if (p > q) { int t = p; p = q; q = t; }
+                double c = 1.0, s = 0.0;
+                if (q < n) {
+                    const double apq = A[(long long)p * n + q];
+                    const double app = A[(long long)p * n + p], aqq = A[(long long)q * n + q];
+                    if (fabs(apq) > 1e-300 && fabs(apq) > 1e-18 * sqrt(fabs(app * aqq))) {
+                        const double tau = (aqq - app) / (2.0 * apq);
+                        const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                        c = 1.0 / sqrt(1.0 + t * t);
+                        s = t * c;
+                    }
+                } else {
+                    q = -1;
+                }
+                cc[k] = c, ss[k] = s, pp[k] = p, qq[k] = q;
+            }
+            __syncthreads();
+            // rows: A <- J^T A
+            for (int e = threadIdx.x; e < npair * n; e += NT) {
+                const int k = e / n, col = e - k * n;
+                const int p = pp[k], q = qq[k];
+                if (q < 0 || ss[k] == 0.0) continue;
+                const double c = cc[k], s = ss[k];
+                const double ap = A[(long long)p * n + col], aq = A[(long long)q * n + col];
+                A[(long long)p * n + col] = c * ap - s * aq;
+                A[(long long)q * n + col] = s * ap + c * aq;
+            }
+            __syncthreads();
+            // columns: A <- A J, Q <- Q J
+            for (int e = threadIdx.x; e < npair * n; e += NT) {
+                const int k = e / n, row = e - k * n;
+                const int p = pp[k], q = qq[k];
+                if (q < 0 || ss[k] == 0.0) continue;
+                const double c = cc[k], s = ss[k];
+                double ap = A[(long long)row * n + p], aq = A[(long long)row * n + q];
+                A[(long long)row * n + p] = c * ap - s * aq;
+                A[(long long)row * n + q] = s * ap + c * aq;
+                ap = Q[(long long)row * n + p], aq = Q[(long long)row * n + q];
+                Q[(long long)row * n + p] = c * ap - s * aq;
+                Q[(long long)row * n + q] = s * ap + c * aq;
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = threadIdx.x; i < n; i += NT) lam[i] = A[(long long)i * n + i];
+    if (ax == 1)
+        for (int e = threadIdx.x; e < n * n; e += NT) {
+            const int i = e / n, j = e - i * n;
+            E.QvT[(long long)j * n + i] = Q[e];
+        }
+}
+
+// ADMM NNLS (osqp.mojo:93-157): rho 0.1, sigma 1e-6, lambda 1e-6, eps 1e-8,
+// at most 50000 iterations; then q0 = D z, clip at 0, normalise by m.q0
+// (initial_cond.mojo:119-129).  grid: n_inst.
+__global__ void __launch_bounds__(NT) k_admm(const Inst* __restrict__ inst, const double* __restrict__ ST,
+                                             const double* __restrict__ EG, double* __restrict__ VC,
+                                             InstOut* __restrict__ out) {
+    const Inst& in = inst[blockIdx.x];
+    const int ns = in.nsb, nv = in.nvb, N = ns * nv;
+    EigView E(const_cast<double*>(EG) + in.o_eig, ns, nv);
+    double* base = VC + in.o_vec;
+    const double* Di = base + (long long)V_DINV * N;
+    const double* bt = base + (long long)V_BT * N;
+    const double* mw = base + (long long)V_MW * N;
+    double* x = base + (long long)V_ADMM_X * N;
+    double* z = base + (long long)V_ADMM_Z * N;
+    double* u = base + (long long)V_ADMM_U * N;
+    double* q = base + (long long)V_ADMM_Q * N;
+    double* R = base + (long long)V_ADMM_R * N;
+    double* T = base + (long long)V_ADMM_T * N;
+    double* q0 = base + (long long)V_Q0 * N;
+    const double* Ms = ST + in.o_st;
+    const double rho = 0.1, sigma = 1e-6, lam = 1e-6, eps_abs = 1e-8, eps_rel = 1e-8;
+    const double alpha = lam + sigma + rho;
+    __shared__ double red[5 * NWAVE];
+    // q = A^T b~ (A symmetric)
+    for (int r = threadIdx.x; r < N; r += NT) {
+        const int I = r / nv, i = r - I * nv;
+        double acc = 0.0;
+        for (int dJ = -3; dJ <= 3; ++dJ) {
+            const int J = I + dJ;
+            if (J < 0 || J >= ns) continue;
+            for (int dj = -3; dj <= 3; ++dj) {
+                const int jj = i + dj;
+                if (jj < 0 || jj >= nv) continue;
+                acc += Ms[(long long)((dJ + 3) * 7 + dj + 3) * N + r] * bt[J * nv + jj];
+            }
+        }
+        q[r] = acc;
+        x[r] = 0.0, z[r] = 0.0, u[r] = 0.0;
+    }
+    __syncthreads();
+    int it = 0;
+    const int max_iter = 50000;
+    for (; it < max_iter;) {
+        for (int r = threadIdx.x; r < N; r += NT) R[r] = q[r] + rho * (z[r] - u[r]) + sigma * x[r];
+        __syncthreads();
+        // T = Qs^T R
+        for (int e = threadIdx.x; e < N; e += NT) {
+            const int a = e / nv, i = e - a * nv;
+            double acc = 0.0;
+            for (int I = 0; I < ns; ++I) acc += E.Qs[(long long)I * ns + a] * R[(long long)I * nv + i];
+            T[e] = acc;
+        }
+        __syncthreads();
+        // R = (T Qv) ./ (ls^2 lv^2 + alpha)
+        for (int e = threadIdx.x; e < N; e += NT) {
+            const int a = e / nv, b = e - a * nv;
+            double acc = 0.0;
+            for (int i = 0; i < nv; ++i) acc += T[(long long)a * nv + i] * E.Qv[(long long)i * nv + b];
+            const double la = E.ls[a] * E.ls[a], lb = E.lv[b] * E.lv[b];
+            R[e] = acc / (la * lb + alpha);
+        }
+        __syncthreads();
+        // T = Qs R
+        for (int e = threadIdx.x; e < N; e += NT) {
+            const int I = e / nv, b = e - I * nv;
+            double acc = 0.0;
+            for (int a = 0; a < ns; ++a) acc += E.Qs[(long long)I * ns + a] * R[(long long)a * nv + b];
+            T[e] = acc;
+        }
+        __syncthreads();
+        // x = T Qv^T ; z, u updates and the five norms
+        double nr[5] = {0, 0, 0, 0, 0};
+        for (int e = threadIdx.x; e < N; e += NT) {
+            const int I = e / nv, i = e - I * nv;
+            double acc = 0.0;
+            for (int b = 0; b < nv; ++b) acc += T[(long long)I * nv + b] * E.QvT[(long long)b * nv + i];
+            const double xv = acc;
+            x[e] = xv;
+            const double zo = z[e];
+            double zn = xv + u[e];
+            if (zn < 0.0) zn = 0.0;
+            z[e] = zn;
+            const double un = u[e] + xv - zn;
+            u[e] = un;
+            const double pv = xv - zn, dv = rho * (zn - zo);
+            nr[0] += pv * pv;
+            nr[1] += dv * dv;
+            nr[2] += zn * zn;
+            nr[3] += xv * xv;
+            nr[4] += un * un;
+        }
+        block_sum_n<5>(nr, red);
+        ++it;
+        const double ep = eps_abs + eps_rel * sqrt(fmax(nr[2], nr[3]));
+        const double ed = eps_abs + eps_rel * rho * sqrt(nr[4]);
+        if (sqrt(nr[0]) < ep && sqrt(nr[1]) < ed) break;
+    }
+    __syncthreads();
+    double md = 0.0;
+    for (int r = threadIdx.x; r < N; r += NT) {
+        double v = Di[r] * z[r];
+        if (v < 0.0) v = 0.0;
+        q0[r] = v;
+        md += mw[r] * v;
+    }
+    md = block_sum(md, red);
+    if (md > 0.0)
+        for (int r = threadIdx.x; r < N; r += NT) q0[r] = q0[r] / md;
+    if (threadIdx.x == 0) out[blockIdx.x].n_admm = it;
+}
+
+}  // namespace fpe

@@ -1,0 +1,214 @@
+/* fpe_engine.h - C ABI of the MI355X Heston Fokker-Planck pricing engine.
+ *
+ * Drop-in replacement for the reference engine's C ABI
+ * (elevenwang-creator/FPE_option cpp/include/fpe_engine.h:17-198, implemented in
+ * src/bindings/c_abi.mojo:109-494): same symbols, same struct layouts, same
+ * ownership rules (the engine allocates every result array; the caller frees it
+ * with the matching fpe_compute_free_* call) and the same failure convention
+ * (NULL handle / {NULL, 0} results). Numerics run in HIP kernels on the GPU.
+ *
+ * Additive extensions (no counterpart in the reference): batched pricing of
+ * many independent options in one call, per-option status codes, solver
+ * options (tolerances, discretisation, complex stage-solve mode, precision)
+ * and a last-error query.
+ */
+#ifndef FPE_ENGINE_H
+#define FPE_ENGINE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/** Opaque handle to an FPE compute pipeline. */
+typedef struct FpeCompute FpeCompute;
+
+/** 1-D array result. Replaces fpe_engine.h:21-24. */
+struct FpeVecResult {
+    double* data;
+    int32_t len;
+};
+
+/** Two 1-D arrays (s and v). Replaces fpe_engine.h:27-32. */
+struct FpeVec2Result {
+    double* s_data;
+    int32_t s_len;
+    double* v_data;
+    int32_t v_len;
+};
+
+/** Grid points and weights in s and v. Replaces fpe_engine.h:35-42. */
+struct FpeGridPtsResult {
+    double* s_data;
+    int32_t s_len;
+    double* v_data;
+    int32_t v_len;
+    double* sw_data;
+    double* vw_data;
+};
+
+/** Row-major 2-D matrix. Replaces fpe_engine.h:45-49. */
+struct FpeMatResult {
+    double* data;
+    int32_t n_rows;
+    int32_t n_cols;
+};
+
+/** Delta, gamma, vega per strike. Replaces fpe_engine.h:52-57. */
+struct FpeGreeksResult {
+    double* delta;
+    double* gamma;
+    double* vega;
+    int32_t len;
+};
+
+/** Price + Greeks per strike. Replaces fpe_engine.h:60-66. */
+struct FpeOneshotResult {
+    double* price;
+    double* delta;
+    double* gamma;
+    double* vega;
+    int32_t len;
+};
+
+/** Create a pipeline; NULL on invalid parameters.
+ *  Replaces fpe_engine.h:87-94 (c_abi.mojo:109-160): s_min < 0 -> 0,
+ *  s_max <= 0 -> 3*S0, v in [0, 1], validity of the type/barrier pair. */
+FpeCompute* fpe_compute_create(double kappa, double theta, double sigma, double rho, double r,
+                               double T, double S0, double V0, int32_t n_s, int32_t n_v,
+                               double barrier, int32_t option_type, int32_t num_insert,
+                               double s_min, double s_max);
+
+/** Replaces fpe_engine.h:97 (c_abi.mojo:163-170). */
+void fpe_compute_destroy(FpeCompute* ptr);
+
+/** Knot vectors. Replaces fpe_engine.h:103 (c_abi.mojo:173-197). */
+void fpe_compute_knots(FpeCompute* ptr, struct FpeVec2Result* result);
+
+/** Quadrature points and weights. Replaces fpe_engine.h:109 (c_abi.mojo:200-230). */
+void fpe_compute_grid_points(FpeCompute* ptr, struct FpeGridPtsResult* result);
+
+/** Initial-condition coefficients. Replaces fpe_engine.h:115 (c_abi.mojo:233-247). */
+void fpe_compute_initial_condition(FpeCompute* ptr, struct FpeVecResult* result);
+
+/** All accepted Radau states (rows) x dofs. Replaces fpe_engine.h:121 (c_abi.mojo:250-268). */
+void fpe_compute_solve(FpeCompute* ptr, struct FpeMatResult* result);
+
+/** Terminal PDF on the quadrature grid. Replaces fpe_engine.h:127 (c_abi.mojo:271-289). */
+void fpe_compute_pdf(FpeCompute* ptr, struct FpeMatResult* result);
+
+/** Discounted prices at strikes. Replaces fpe_engine.h:135-140 (c_abi.mojo:292-311). */
+void fpe_compute_price(FpeCompute* ptr, const double* K, int32_t n_K, struct FpeVecResult* result);
+
+/** Finite-difference Greeks. Replaces fpe_engine.h:150-157 (c_abi.mojo:314-353).
+ *  As in the reference, rel_s / rel_v are used as given (0 is not replaced). */
+void fpe_compute_greeks(FpeCompute* ptr, const double* K, int32_t n_K, double rel_s, double rel_v,
+                        struct FpeGreeksResult* result);
+
+/** One-shot price + Greeks. Replaces fpe_engine.h:164-173 (c_abi.mojo:356-440). */
+void fpe_price_oneshot(double kappa, double theta, double sigma, double rho, double r, double T,
+                       double S0, double V0, const double* K, int32_t n_K, double barrier,
+                       int32_t option_type, int32_t n_s, int32_t n_v, int32_t num_insert,
+                       double s_min, double s_max, struct FpeOneshotResult* result);
+
+/** Free functions. Replace fpe_engine.h:176-186 (c_abi.mojo:443-494). */
+void fpe_compute_free_vec(struct FpeVecResult* r);
+void fpe_compute_free_vec2(struct FpeVec2Result* r);
+void fpe_compute_free_grid_pts(struct FpeGridPtsResult* r);
+void fpe_compute_free_mat(struct FpeMatResult* r);
+void fpe_compute_free_greeks(struct FpeGreeksResult* r);
+void fpe_compute_free_oneshot(struct FpeOneshotResult* r);
+
+/* ------------------------------------------------------------------ extensions */
+
+/* Discretisation modes. */
+#define FPE_DISC_REFERENCE 0 /* knot generator + insert-grid quadrature (reference) */
+#define FPE_DISC_SPAN 1      /* n_s x n_v dofs, sinh-clustered knots, 4-pt GL per span */
+
+/* Complex Radau stage-system solve. */
+#define FPE_CPLX_GMRES 0  /* GMRES(10) preconditioned by the real LU (reference) */
+#define FPE_CPLX_DIRECT 1 /* complex block-banded LU of (ALPH + i BETA) M + h K */
+
+/* Factorisation precision. */
+#define FPE_PREC_FP64 0
+#define FPE_PREC_MIXED 1 /* FP32 block LU + FP64 iterative refinement */
+
+/* Per-option status codes. */
+#define FPE_OK 0
+#define FPE_INVALID 1      /* parameters rejected by the reference validity rules */
+#define FPE_MAX_STEPS 2    /* Radau exceeded 100000 steps */
+#define FPE_H_UNDERFLOW 3  /* step size fell below 1e-14 */
+#define FPE_NONFINITE 4    /* NaN/Inf in the solution */
+#define FPE_TOO_LARGE 5    /* grid exceeds the kernels' limits */
+
+/** One option of a batch: parameters, one strike, type and solver options. */
+typedef struct FpeBatchOption {
+    double kappa, theta, sigma, rho, r, T, S0, V0;
+    double s_min, s_max;  /* s_min < 0 -> 0, s_max <= 0 -> 3*S0 */
+    double v_max;         /* <= 0 -> 1 (reference) */
+    double barrier;
+    double strike;
+    int32_t option_type;  /* 0..9 */
+    int32_t n_s, n_v;     /* reference: knot-generator input; span: dof counts */
+    int32_t disc_mode;    /* FPE_DISC_* */
+    int32_t num_insert;   /* reference mode */
+    int32_t complex_mode; /* FPE_CPLX_* */
+    int32_t precision;    /* FPE_PREC_* */
+    int32_t exact_constants;
+    double ic_sigma_s, ic_sigma_v; /* <= 0 -> reference */
+    double rtol, atol;             /* <= 0 -> 1e-4 / 1e-6 */
+    double knot_width_s, knot_width_v; /* span mode, <= 0 -> 0.1 */
+} FpeBatchOption;
+
+/** Per-batch counters (summed over the solves of the batch). */
+typedef struct FpeBatchStats {
+    int64_t n_solves, n_accepted, n_rejected, n_lu, n_newton, n_err_solves, n_inner_iters,
+        n_admm_iters;
+    double ms_setup, ms_assemble, ms_ic, ms_radau, ms_payoff, ms_total;
+} FpeBatchStats;
+
+/** CSR matrix (row-major, sorted columns). */
+struct FpeCsrResult {
+    double* data;
+    int32_t* indices;
+    int32_t* indptr;
+    int32_t n_rows, n_cols, nnz;
+};
+
+/** Recombined cubic B-spline values (deriv = 0) or first derivatives (deriv = 1)
+ *  at the quadrature points of axis 0 = s or 1 = v, evaluated on the GPU.
+ *  Serves the reference's Python-only Compute.basis_1d (_py_pipeline.mojo:46-57,
+ *  compute_pipeline.mojo:63-71). Entries that are exactly zero are omitted. */
+void fpe_compute_basis_1d(FpeCompute* ptr, int32_t axis, int32_t deriv, struct FpeCsrResult* result);
+void fpe_compute_free_csr(struct FpeCsrResult* r);
+
+/** Fill an FpeBatchOption with the reference defaults. */
+void fpe_batch_option_default(FpeBatchOption* o);
+
+/** Price n independent options on the GPU (in-options via in-out parity).
+ *  prices[i] is the discounted price; status[i] an FPE_* code.
+ *  Returns 0 on success, nonzero if the batch could not run (see fpe_last_error). */
+int32_t fpe_batch_price(const FpeBatchOption* opts, int32_t n, double* prices, int32_t* status,
+                        FpeBatchStats* stats);
+
+/** Human-readable description of the last failure in this thread (empty if none). */
+const char* fpe_last_error(void);
+
+/** Number of usable GPUs (0 if the HIP runtime finds none). */
+int32_t fpe_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#ifdef __cplusplus
+static_assert(sizeof(struct FpeVecResult) == 16, "FpeVecResult layout mismatch");
+static_assert(sizeof(struct FpeVec2Result) == 32, "FpeVec2Result layout mismatch");
+static_assert(sizeof(struct FpeGridPtsResult) == 48, "FpeGridPtsResult layout mismatch");
+static_assert(sizeof(struct FpeMatResult) == 16, "FpeMatResult layout mismatch");
+static_assert(sizeof(struct FpeGreeksResult) == 32, "FpeGreeksResult layout mismatch");
+static_assert(sizeof(struct FpeOneshotResult) == 40, "FpeOneshotResult layout mismatch");
+#endif
+
+#endif
