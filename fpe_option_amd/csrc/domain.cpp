@@ -30,12 +30,16 @@ std::vector<double> unit_scale(const std::vector<double>& x) {
     return y;
 }
 
-// 4-point Gauss-Legendre rule on [-1,1] (src/numerics/bspline/knots.mojo:33-39).
+// 4-point rules on [-1,1]. exact = false: the reference's rule
+// (src/numerics/bspline/knots.mojo:33-39), whose nodes use sqrt(5/6) where the
+// Gauss-Legendre nodes need sqrt(6/5); kept for reference parity. exact = true:
+// the Gauss-Legendre rule, used by the span discretisation.
 struct GL4 {
     double x[4], w[4];
-    GL4() {
-        const double a = std::sqrt(3.0 / 7.0 + 2.0 / 7.0 * std::sqrt(5.0 / 6.0));
-        const double b = std::sqrt(3.0 / 7.0 - 2.0 / 7.0 * std::sqrt(5.0 / 6.0));
+    explicit GL4(bool exact) {
+        const double r = exact ? std::sqrt(6.0 / 5.0) : std::sqrt(5.0 / 6.0);
+        const double a = std::sqrt(3.0 / 7.0 + 2.0 / 7.0 * r);
+        const double b = std::sqrt(3.0 / 7.0 - 2.0 / 7.0 * r);
         const double wa = (18.0 - std::sqrt(30.0)) / 36.0, wb = (18.0 + std::sqrt(30.0)) / 36.0;
         x[0] = -a, x[1] = -b, x[2] = b, x[3] = a;
         w[0] = wa, w[1] = wb, w[2] = wb, w[3] = wa;
@@ -126,11 +130,14 @@ std::vector<double> insert_grid(double c, double sd, double lo, double hi, int n
     return g;
 }
 
-// Gauss points of consecutive intervals; with_left adds each interval's left end
-// with weight 0 (src/engines/fpe/domain.mojo:79-111).
+// Quadrature points of consecutive intervals. with_left (reference
+// discretisation): each interval's left end with weight 0 followed by the
+// reference's 4-point rule (src/engines/fpe/domain.mojo:79-111); otherwise the
+// 4-point Gauss-Legendre rule.
 void interval_quadrature(const std::vector<double>& g, bool with_left, std::vector<double>& x,
                          std::vector<double>& w) {
-    static const GL4 gl;
+    static const GL4 ref_rule(false), gauss_rule(true);
+    const GL4& gl = with_left ? ref_rule : gauss_rule;
     x.clear();
     w.clear();
     for (size_t k = 0; k + 1 < g.size(); ++k) {

@@ -33,7 +33,6 @@ namespace fpe {
 #define RC_T31 9.6604818261509293619e-01
 #define RC_TI11 4.3255798900631553510e00
 #define RC_TI12 3.3919925181580986954e-01
-#define RC_TI13 5.4177705399358748719e-01
 #define RC_TI21 -4.1787185915519047273e00
 #define RC_TI22 -3.2768282076106238708e-01
 #define RC_TI23 4.7662355450055045196e-01
@@ -178,6 +177,9 @@ __global__ void __launch_bounds__(NT) k_radau(const Inst* __restrict__ inst, int
         const double U1 = in.exact ? 3.637834252744496 : 3.6378165692476072;
         const double ALPH = in.exact ? 2.681082873627752 : 2.6753213032678365;
         const double BETA = in.exact ? 3.050430199247411 : 3.0493570545593676;
+        // radau.mojo:73 TI13 is a digit transposition of RADAU5's 0.54177053993587487119;
+        // kept in reference mode, corrected with the exact constants.
+        const double RC_TI13 = in.exact ? 5.4177053993587487119e-01 : 5.4177705399358748719e-01;
 
         double* y = vv(S_Y);
         for (int r = tid; r < N; r += NT) y[r] = D[r] * q0[r];

@@ -208,10 +208,15 @@ void quad_reference(const vec& grid, vec& pts, vec& wts) {
     }
 }
 
-// Span mode: 4 GL nodes per knot span, no zero-weight endpoint.
+// Span mode: the true 4-point Gauss-Legendre rule per knot span (nodes
+// sqrt(3/7 -+ 2/7 sqrt(6/5))), no zero-weight endpoint. The reference's 4-point
+// rule (knots.mojo:33-39) uses sqrt(5/6) in the nodes and is kept for the
+// reference discretisation only.
 void quad_span(const vec& brk, vec& pts, vec& wts) {
-    vec nd, wt;
-    gauss_legendre(4, nd, wt);
+    const double a = std::sqrt(3.0 / 7.0 + 2.0 / 7.0 * std::sqrt(6.0 / 5.0));
+    const double b = std::sqrt(3.0 / 7.0 - 2.0 / 7.0 * std::sqrt(6.0 / 5.0));
+    const double wa = (18.0 - std::sqrt(30.0)) / 36.0, wb = (18.0 + std::sqrt(30.0)) / 36.0;
+    vec nd = {-a, -b, b, a}, wt = {wa, wb, wb, wa};
     pts.clear();
     wts.clear();
     for (size_t i = 0; i + 1 < brk.size(); ++i) {
@@ -446,12 +451,16 @@ const double DD1 = (-13.0 - 7.0 * SQ6) / 3.0, DD2 = (-13.0 + 7.0 * SQ6) / 3.0, D
 const double REF_U1 = 3.6378165692476072, REF_ALPH = 2.6753213032678365,
              REF_BETA = 3.0493570545593676;
 const double EX_U1 = 3.637834252744496, EX_ALPH = 2.681082873627752, EX_BETA = 3.050430199247411;
+// radau.mojo:73 has TI13 = 0.54177705399358748719, a digit transposition of
+// Hairer's RADAU5 value 0.54177053993587487119 (= inv(T)[0][2]); with it T*TI != I
+// by ~6e-6 and Newton converges to a perturbed stage system. Reference mode keeps it.
+const double EX_TI13 = 5.4177053993587487119e-01;
 const double T11 = 9.1232394870892942792e-02, T12 = -1.4125529502095420843e-01,
              T13 = -3.0029194105147424492e-02, T21 = 2.4171793270710701896e-01,
              T22 = 2.0412935229379993199e-01, T23 = 3.8294211275726193779e-01,
              T31 = 9.6604818261509293619e-01;
 const double TI11 = 4.3255798900631553510e00, TI12 = 3.3919925181580986954e-01,
-             TI13 = 5.4177705399358748719e-01, TI21 = -4.1787185915519047273e00,
+             TI13_REF = 5.4177705399358748719e-01, TI21 = -4.1787185915519047273e00,
              TI22 = -3.2768282076106238708e-01, TI23 = 4.7662355450055045196e-01,
              TI31 = -5.0287263494578687595e-01, TI32 = 2.5719269498556054292e00,
              TI33 = -5.9603920482822492497e-01;
@@ -491,7 +500,7 @@ RadauOut radau_solve(const Sys& S, const vec& y0, double t1, double rtol, double
     const CSR& M = S.M;
     const CSR& K = S.K;
     const double U1 = exact ? EX_U1 : REF_U1, ALPH = exact ? EX_ALPH : REF_ALPH,
-                 BETA = exact ? EX_BETA : REF_BETA;
+                 BETA = exact ? EX_BETA : REF_BETA, TI13 = exact ? EX_TI13 : TI13_REF;
     RadauOut out;
     double t0 = 0.0, t = t0, posneg = 1.0;
     out.ts.push_back(t0);
