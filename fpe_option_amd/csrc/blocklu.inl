@@ -26,8 +26,10 @@ struct LuLds {
 
 // C (op)= A B over an M x N x K product with element accessors; each wave
 // computes 16x16 output tiles with v_mfma_f64_16x16x4f64.
-// Lane layout (verified on gfx950): a = A[row0 + lane%16][k0 + lane/16],
-// b = B[k0 + lane/16][col0 + lane%16], acc[j] = C[row0 + 4*(lane/16) + j][col0 + lane%16].
+// Lane layout of v_mfma_f64_16x16x4f64 (checked on gfx950 with an in-bounds
+// probe): a = A[row0 + lane%16][k0 + lane/16], b = B[k0 + lane/16][col0 + lane%16],
+// acc[j] = C[row0 + lane/16 + 4*j][col0 + lane%16]. (The f32 16x16x4 form places
+// acc[j] at row 4*(lane/16) + j instead.)
 template <bool SUB, class FA, class FB, class FC>
 __device__ __forceinline__ void mfma_gemm(int M, int N, int K, FA A, FB B, FC C) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -47,7 +49,7 @@ __device__ __forceinline__ void mfma_gemm(int M, int N, int K, FA A, FB B, FC C)
         if (cb < N) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const int r = r0 + 4 * l4 + j;
+                const int r = r0 + l4 + 4 * j;
                 if (r < M) {
                     if (SUB) C(r, cb) -= acc[j];
                     else C(r, cb) = acc[j];
@@ -82,7 +84,7 @@ __device__ __forceinline__ void mfma_gemm_c(int M, int N, int K, FAr Ar, FAi Ai,
         if (cb < N) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const int r = r0 + 4 * l4 + j;
+                const int r = r0 + l4 + 4 * j;
                 if (r < M) {
                     if (SUB) {
                         Cr(r, cb) -= accr[j];

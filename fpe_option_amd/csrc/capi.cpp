@@ -361,6 +361,19 @@ int32_t fpe_batch_price(const FpeBatchOption* opts, int32_t n, double* prices, i
             stats->n_newton += r.n_newt, stats->n_err_solves += r.n_err, stats->n_inner_iters += r.n_inner;
             stats->n_admm_iters += r.n_admm;
         }
+        for (auto& r : res) {
+            const double nb = r.nsb, m = r.nvb, N = nb * m;
+            const double lu = 24.67 * nb * m * m * m, sol = 14.0 * nb * m * m, spmv = 98.0 * N;
+            const double fac_bytes = 7.0 * nb * m * m * 8.0;
+            // direct: real + complex LU per refresh; GMRES: real LU and the solves of the inner iterations
+            const bool direct = r.direct;
+            stats->alg_flops += r.n_lu * (direct ? 5.0 : 1.0) * lu +
+                                r.n_newt * ((direct ? 5.0 : 3.0) * sol + 6.0 * spmv) +
+                                r.n_inner * (2.0 * sol + 4.0 * spmv) + r.n_err * (sol + 2.0 * spmv);
+            stats->alg_bytes += r.n_lu * (direct ? 3.0 : 1.0) * 2.0 * fac_bytes +
+                                r.n_newt * (direct ? 3.0 : 1.0) * fac_bytes + r.n_inner * fac_bytes +
+                                r.n_err * fac_bytes;
+        }
         stats->ms_setup = bt.setup, stats->ms_assemble = bt.assemble, stats->ms_ic = bt.ic;
         stats->ms_radau = bt.radau, stats->ms_payoff = bt.payoff, stats->ms_total = bt.total;
     }

@@ -264,6 +264,7 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
         R.status = o.status, R.n_acc = o.n_acc, R.n_rej = o.n_rej, R.n_lu = o.n_lu, R.n_newt = o.n_newt;
         R.n_err = o.n_err, R.n_inner = o.n_inner, R.n_admm = o.n_admm;
         R.nsb = hi[s].nsb, R.nvb = hi[s].nvb;
+        R.direct = hi[s].cmode == 1;
         const long long N = (long long)hi[s].nsb * hi[s].nvb;
         if (opt.want_basis) {
             for (int ax = 0; ax < 2; ++ax) {

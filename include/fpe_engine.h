@@ -166,6 +166,11 @@ typedef struct FpeBatchStats {
     int64_t n_solves, n_accepted, n_rejected, n_lu, n_newton, n_err_solves, n_inner_iters,
         n_admm_iters;
     double ms_setup, ms_assemble, ms_ic, ms_radau, ms_payoff, ms_total;
+    /* Algorithmic work of the Radau kernel, summed over solves: block-banded LU
+     * 24.67 n_sb n_vb^3 flops per real factorisation (x4 complex), banded solve
+     * 14 n_sb n_vb^2 (x4 complex), 49-point SpMV 98 N; bytes = factor panels read
+     * by every solve (7 n_sb n_vb^2 doubles real, x2 complex) plus factor writes. */
+    double alg_flops, alg_bytes;
 } FpeBatchStats;
 
 /** CSR matrix (row-major, sorted columns). */
