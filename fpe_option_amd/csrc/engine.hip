@@ -69,6 +69,8 @@ bool dev_init(std::string& err) {
     }
     HIP_OK(hipSetDevice(0));
     HIP_OK(hipDeviceGetAttribute(&g_dev.n_cu, hipDeviceAttributeMultiprocessorCount, 0));
+    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_admm), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               int(ADMM_LDS_MAX)));
     HIP_OK(hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking));
     for (auto& e : g_dev.ev) HIP_OK(hipEventCreate(&e));
     g_dev.init = true;
@@ -120,6 +122,10 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
         if (I.nvb < 4 || I.nvb > MAX_BLOCK || I.nsb < 4 || I.nsb > 300) {
             err = "grid too large for the kernels (n_vb must be in [4, 64], n_sb in [4, 300]): n_sb=" +
                   std::to_string(I.nsb) + " n_vb=" + std::to_string(I.nvb);
+            return false;
+        }
+        if (admm_lds_bytes(I.nsb, I.nvb, false) > ADMM_LDS_MAX) {
+            err = "grid too large for the initial-condition kernel's LDS budget";
             return false;
         }
         I.s_lneu = sp.s_left_neumann, I.s_rneu = sp.s_right_neumann;
@@ -222,7 +228,7 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
     hipLaunchKernelGGL(k_ic_rhs, dim3(n), dim3(NT), 0, st, dI, G.qd.as<double>(), G.bv.as<double>(),
                        G.bc.as<int>(), G.vc.as<double>());
     hipLaunchKernelGGL(k_eig, dim3(n, 2), dim3(NT), 0, st, dI, G.fc.as<double>(), G.eg.as<double>());
-    hipLaunchKernelGGL(k_admm, dim3(n), dim3(NT), 0, st, dI, G.st.as<double>(), G.eg.as<double>(),
+    hipLaunchKernelGGL(k_admm, dim3(n), dim3(NT), ADMM_LDS_MAX, st, dI, G.st.as<double>(), G.eg.as<double>(),
                        G.vc.as<double>(), G.out.as<InstOut>());
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord(G.ev[2], st));

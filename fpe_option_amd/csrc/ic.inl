@@ -188,12 +188,50 @@ __global__ void __launch_bounds__(NT) k_eig(const Inst* __restrict__ inst, const
         }
 }
 
+// C = A B with a K-loop (operands typically in LDS), v_mfma_f64_16x16x4f64;
+// C(r, c, value) receives each output once.
+template <class FA, class FB, class FC>
+__device__ __forceinline__ void mfma_gemm_k(int M, int N, int K, FA A, FB B, FC C) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tm = (M + 15) >> 4, tn = (N + 15) >> 4;
+    const int l16 = lane & 15, l4 = lane >> 4;
+    for (int t = wave; t < tm * tn; t += NWAVE) {
+        const int r0 = (t / tn) << 4, c0 = (t % tn) << 4;
+        const int ra = r0 + l16, cb = c0 + l16;
+        d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+        for (int k0 = 0; k0 < K; k0 += 4) {
+            const int k = k0 + l4;
+            const double a = (ra < M && k < K) ? A(ra, k) : 0.0;
+            const double b = (cb < N && k < K) ? B(k, cb) : 0.0;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        }
+        if (cb < N) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int r = r0 + l4 + 4 * j;
+                if (r < M) C(r, cb, acc[j]);
+            }
+        }
+    }
+}
+
+// Bytes of dynamic LDS k_admm needs for an ns x nv instance, with (qs_in_lds)
+// or without the ns x ns eigenvector matrix.
+__host__ __device__ inline long long admm_lds_bytes(int ns, int nv, bool qs_in_lds) {
+    return 8LL * ((long long)nv * (nv + 1) + 2LL * ns * (nv + 1) + (qs_in_lds ? (long long)ns * (ns + 1) : 0));
+}
+constexpr long long ADMM_LDS_MAX = 150 * 1024;
+
 // ADMM NNLS (osqp.mojo:93-157): rho 0.1, sigma 1e-6, lambda 1e-6, eps 1e-8,
 // at most 50000 iterations; then q0 = D z, clip at 0, normalise by m.q0
-// (initial_cond.mojo:119-129).  grid: n_inst.
+// (initial_cond.mojo:119-129). The four products per iteration run on MFMA
+// with Qv and the two ns x nv work matrices in LDS (Qs too when it fits).
+// grid: n_inst; dynamic LDS: ADMM_LDS_MAX.
 __global__ void __launch_bounds__(NT) k_admm(const Inst* __restrict__ inst, const double* __restrict__ ST,
                                              const double* __restrict__ EG, double* __restrict__ VC,
                                              InstOut* __restrict__ out) {
+    extern __shared__ double dyn[];
     const Inst& in = inst[blockIdx.x];
     const int ns = in.nsb, nv = in.nvb, N = ns * nv;
     EigView E(const_cast<double*>(EG) + in.o_eig, ns, nv);
@@ -205,13 +243,22 @@ __global__ void __launch_bounds__(NT) k_admm(const Inst* __restrict__ inst, cons
     double* z = base + (long long)V_ADMM_Z * N;
     double* u = base + (long long)V_ADMM_U * N;
     double* q = base + (long long)V_ADMM_Q * N;
-    double* R = base + (long long)V_ADMM_R * N;
-    double* T = base + (long long)V_ADMM_T * N;
     double* q0 = base + (long long)V_Q0 * N;
     const double* Ms = ST + in.o_st;
     const double rho = 0.1, sigma = 1e-6, lam = 1e-6, eps_abs = 1e-8, eps_rel = 1e-8;
     const double alpha = lam + sigma + rho;
     __shared__ double red[5 * NWAVE];
+    // LDS: Qv (nv x nv+1), T and R (ns x nv+1), Qs (ns x ns+1) when it fits
+    const int lv = nv + 1, lsd = ns + 1;
+    double* sQv = dyn;
+    double* sT = sQv + nv * lv;
+    double* sR = sT + ns * lv;
+    const bool qs_lds = admm_lds_bytes(ns, nv, true) <= ADMM_LDS_MAX;
+    double* sQs = sR + ns * lv;
+    for (int e = threadIdx.x; e < nv * nv; e += NT) sQv[(e / nv) * lv + e % nv] = E.Qv[e];
+    if (qs_lds)
+        for (int e = threadIdx.x; e < ns * ns; e += NT) sQs[(e / ns) * lsd + e % ns] = E.Qs[e];
+    auto Qs = [&](int I, int a) -> double { return qs_lds ? sQs[I * lsd + a] : E.Qs[(long long)I * ns + a]; };
     // q = A^T b~ (A symmetric)
     for (int r = threadIdx.x; r < N; r += NT) {
         const int I = r / nv, i = r - I * nv;
@@ -232,54 +279,44 @@ __global__ void __launch_bounds__(NT) k_admm(const Inst* __restrict__ inst, cons
     int it = 0;
     const int max_iter = 50000;
     for (; it < max_iter;) {
-        for (int r = threadIdx.x; r < N; r += NT) R[r] = q[r] + rho * (z[r] - u[r]) + sigma * x[r];
+        for (int r = threadIdx.x; r < N; r += NT) sR[(r / nv) * lv + r % nv] = q[r] + rho * (z[r] - u[r]) + sigma * x[r];
         __syncthreads();
         // T = Qs^T R
-        for (int e = threadIdx.x; e < N; e += NT) {
-            const int a = e / nv, i = e - a * nv;
-            double acc = 0.0;
-            for (int I = 0; I < ns; ++I) acc += E.Qs[(long long)I * ns + a] * R[(long long)I * nv + i];
-            T[e] = acc;
-        }
+        mfma_gemm_k(ns, nv, ns, [&](int a, int I) { return Qs(I, a); }, [&](int I, int i) { return sR[I * lv + i]; },
+                    [&](int a, int i, double v) { sT[a * lv + i] = v; });
         __syncthreads();
         // R = (T Qv) ./ (ls^2 lv^2 + alpha)
-        for (int e = threadIdx.x; e < N; e += NT) {
-            const int a = e / nv, b = e - a * nv;
-            double acc = 0.0;
-            for (int i = 0; i < nv; ++i) acc += T[(long long)a * nv + i] * E.Qv[(long long)i * nv + b];
-            const double la = E.ls[a] * E.ls[a], lb = E.lv[b] * E.lv[b];
-            R[e] = acc / (la * lb + alpha);
-        }
+        mfma_gemm_k(ns, nv, nv, [&](int a, int i) { return sT[a * lv + i]; },
+                    [&](int i, int b) { return sQv[i * lv + b]; },
+                    [&](int a, int b, double v) {
+                        const double la = E.ls[a] * E.ls[a], lb = E.lv[b] * E.lv[b];
+                        sR[a * lv + b] = v / (la * lb + alpha);
+                    });
         __syncthreads();
         // T = Qs R
-        for (int e = threadIdx.x; e < N; e += NT) {
-            const int I = e / nv, b = e - I * nv;
-            double acc = 0.0;
-            for (int a = 0; a < ns; ++a) acc += E.Qs[(long long)I * ns + a] * R[(long long)a * nv + b];
-            T[e] = acc;
-        }
+        mfma_gemm_k(ns, nv, ns, [&](int I, int a) { return Qs(I, a); }, [&](int a, int b) { return sR[a * lv + b]; },
+                    [&](int I, int b, double v) { sT[I * lv + b] = v; });
         __syncthreads();
-        // x = T Qv^T ; z, u updates and the five norms
+        // x = T Qv^T, then the z and u updates and the five norms
         double nr[5] = {0, 0, 0, 0, 0};
-        for (int e = threadIdx.x; e < N; e += NT) {
-            const int I = e / nv, i = e - I * nv;
-            double acc = 0.0;
-            for (int b = 0; b < nv; ++b) acc += T[(long long)I * nv + b] * E.QvT[(long long)b * nv + i];
-            const double xv = acc;
-            x[e] = xv;
-            const double zo = z[e];
-            double zn = xv + u[e];
-            if (zn < 0.0) zn = 0.0;
-            z[e] = zn;
-            const double un = u[e] + xv - zn;
-            u[e] = un;
-            const double pv = xv - zn, dv = rho * (zn - zo);
-            nr[0] += pv * pv;
-            nr[1] += dv * dv;
-            nr[2] += zn * zn;
-            nr[3] += xv * xv;
-            nr[4] += un * un;
-        }
+        mfma_gemm_k(ns, nv, nv, [&](int I, int b) { return sT[I * lv + b]; },
+                    [&](int b, int i) { return sQv[i * lv + b]; },
+                    [&](int I, int i, double xv) {
+                        const int e = I * nv + i;
+                        x[e] = xv;
+                        const double zo = z[e], uo = u[e];
+                        double zn = xv + uo;
+                        if (zn < 0.0) zn = 0.0;
+                        z[e] = zn;
+                        const double un = uo + xv - zn;
+                        u[e] = un;
+                        const double pv = xv - zn, dv = rho * (zn - zo);
+                        nr[0] += pv * pv;
+                        nr[1] += dv * dv;
+                        nr[2] += zn * zn;
+                        nr[3] += xv * xv;
+                        nr[4] += un * un;
+                    });
         block_sum_n<5>(nr, red);
         ++it;
         const double ep = eps_abs + eps_rel * sqrt(fmax(nr[2], nr[3]));

@@ -60,7 +60,6 @@ struct RadauLds {
     LuLds lu;
     double red[8 * NWAVE];
     double sT[2 * 3 * MAX_BLOCK];
-    int sP[MAX_BLOCK];
     double H[(GM_K + 1) * GM_K];
     double cs[GM_K + 1], sn[GM_K + 1], gs[GM_K + 1], gy[GM_K];
     int inst;
@@ -167,10 +166,6 @@ __global__ void __launch_bounds__(NT) k_radau(const Inst* __restrict__ inst, int
         double* P = ws + L.o_p;
         double* Cr = ws + L.o_cr;
         double* Ci = ws + L.o_ci;
-        double* Tr = ws + L.o_t;
-        double* Ti = Tr + 6LL * m * m;
-        int* piv = wi;
-        int* pivc = wi + (long long)ns * m;
         double* const vb = ws + L.o_v;
         auto vv = [=](int k) { return vb + (long long)k * N; };
         const bool direct = in.cmode == 1;
@@ -218,11 +213,11 @@ __global__ void __launch_bounds__(NT) k_radau(const Inst* __restrict__ inst, int
             if (need_lu) {
                 assemble_E<false>(Ms, Ks, ns, m, U1, 0.0, h, P, nullptr);
                 __syncthreads();
-                int bad = block_lu<false>(P, nullptr, piv, ns, m, Tr, nullptr, S.lu);
+                int bad = block_lu<false>(P, nullptr, ns, m, S.lu);
                 if (direct) {
                     assemble_E<true>(Ms, Ks, ns, m, ALPH, BETA, h, Cr, Ci);
                     __syncthreads();
-                    bad |= block_lu<true>(Cr, Ci, pivc, ns, m, Tr, Ti, S.lu);
+                    bad |= block_lu<true>(Cr, Ci, ns, m, S.lu);
                 }
                 (void)bad;
                 h_lu = h;
@@ -274,14 +269,14 @@ __global__ void __launch_bounds__(NT) k_radau(const Inst* __restrict__ inst, int
                 __syncthreads();
                 if (direct) {
                     double* xr1[1] = {vv(S_DF1)};
-                    block_solve<false, 1>(P, nullptr, piv, ns, m, xr1, nullptr, S.sT, S.sP);
+                    block_solve<false, 1>(P, nullptr, ns, m, xr1, nullptr, S.sT);
                     double* xr2[1] = {vv(S_DF2)};
                     double* xi2[1] = {vv(S_DF3)};
-                    block_solve<true, 1>(Cr, Ci, pivc, ns, m, xr2, xi2, S.sT, S.sP);
+                    block_solve<true, 1>(Cr, Ci, ns, m, xr2, xi2, S.sT);
                 } else {
                     // dF1 and the preconditioned GMRES right-hand side in one pass
                     double* xr3[3] = {vv(S_DF1), vv(S_VRE0), vv(S_VIM0)};
-                    block_solve<false, 3>(P, nullptr, piv, ns, m, xr3, nullptr, S.sT, S.sP);
+                    block_solve<false, 3>(P, nullptr, ns, m, xr3, nullptr, S.sT);
                     // ---- GMRES(10), radau.mojo:916-1146
                     double bsq = 0.0;
                     for (int r = tid; r < N; r += NT) {
@@ -306,7 +301,7 @@ __global__ void __launch_bounds__(NT) k_radau(const Inst* __restrict__ inst, int
                                      vv(S_WIM));
                             __syncthreads();
                             double* xw[2] = {vv(S_WRE), vv(S_WIM)};
-                            block_solve<false, 2>(P, nullptr, piv, ns, m, xw, nullptr, S.sT, S.sP);
+                            block_solve<false, 2>(P, nullptr, ns, m, xw, nullptr, S.sT);
                             for (int k = 0; k <= j; ++k) {
                                 double hv = 0.0;
                                 for (int r = tid; r < N; r += NT)
@@ -447,7 +442,7 @@ __global__ void __launch_bounds__(NT) k_radau(const Inst* __restrict__ inst, int
             __syncthreads();
             {
                 double* xe[1] = {vv(S_ERR)};
-                block_solve<false, 1>(P, nullptr, piv, ns, m, xe, nullptr, S.sT, S.sP);
+                block_solve<false, 1>(P, nullptr, ns, m, xe, nullptr, S.sT);
             }
             ++n_err;
             double en = 0.0;
@@ -466,7 +461,7 @@ __global__ void __launch_bounds__(NT) k_radau(const Inst* __restrict__ inst, int
                 for (int r = tid; r < N; r += NT) vv(S_ERR)[r] = vv(S_MC)[r] - h * vv(S_W)[r];
                 __syncthreads();
                 double* xe[1] = {vv(S_ERR)};
-                block_solve<false, 1>(P, nullptr, piv, ns, m, xe, nullptr, S.sT, S.sP);
+                block_solve<false, 1>(P, nullptr, ns, m, xe, nullptr, S.sT);
                 ++n_err;
                 en = 0.0;
                 for (int r = tid; r < N; r += NT) {
