@@ -64,7 +64,9 @@ class FpeBatchStats(C.Structure):
     _fields_ = [(n, C.c_int64) for n in ("n_solves", "n_accepted", "n_rejected", "n_lu", "n_newton",
                                           "n_err_solves", "n_inner_iters", "n_admm_iters")] + [
         (n, C.c_double) for n in ("ms_setup", "ms_assemble", "ms_ic", "ms_radau", "ms_payoff", "ms_total",
-                                  "alg_flops", "alg_bytes")]
+                                  "alg_flops", "alg_bytes", "wg_s_assemble", "wg_s_lu", "wg_s_solve",
+                                  "wg_s_spmv", "wg_s_other", "wg_s_total", "wg_s_lu_inverse", "wg_s_lu_stage",
+                                  "wg_s_lu_lmul", "wg_s_lu_schur")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

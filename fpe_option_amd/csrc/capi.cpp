@@ -374,6 +374,12 @@ int32_t fpe_batch_price(const FpeBatchOption* opts, int32_t n, double* prices, i
                                 r.n_newt * (direct ? 3.0 : 1.0) * fac_bytes + r.n_inner * fac_bytes +
                                 r.n_err * fac_bytes;
         }
+        for (auto& r : res) {
+            stats->wg_s_assemble += r.tk[0], stats->wg_s_lu += r.tk[1], stats->wg_s_solve += r.tk[2];
+            stats->wg_s_spmv += r.tk[3], stats->wg_s_other += r.tk[4], stats->wg_s_total += r.tk[5];
+            stats->wg_s_lu_inverse += r.tk[6], stats->wg_s_lu_stage += r.tk[7];
+            stats->wg_s_lu_lmul += r.tk[8], stats->wg_s_lu_schur += r.tk[9];
+        }
         stats->ms_setup = bt.setup, stats->ms_assemble = bt.assemble, stats->ms_ic = bt.ic;
         stats->ms_radau = bt.radau, stats->ms_payoff = bt.payoff, stats->ms_total = bt.total;
     }

@@ -42,6 +42,9 @@ struct InstOut {
     int status, n_acc, n_rej, n_lu, n_newt, n_err, n_inner, n_admm;
     int n_hist, pad;
     double t_end, h_last;
+    // wall-clock ticks (100 MHz) spent per phase by the solver workgroup
+    unsigned long long tk_asm, tk_lu, tk_solve, tk_mv, tk_other, tk_total;
+    unsigned long long tk_lu_part[4];  // factorisation: pivot inverse, staging, L multiply, Schur update
 };
 
 // VC vector slots (each N doubles).

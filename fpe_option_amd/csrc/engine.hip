@@ -175,6 +175,7 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
         max_m = std::max(max_m, I.nvb);
         any_direct |= (sp.complex_mode == 1);
     }
+    max_panel = (max_panel + 31) & ~31LL;  // panels start 256-byte aligned (16-byte stores in assembly)
     SlotLayout L{};
     L.o_p = 0;
     L.o_cr = L.o_p + max_panel;
@@ -271,6 +272,9 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
         R.n_err = o.n_err, R.n_inner = o.n_inner, R.n_admm = o.n_admm;
         R.nsb = hi[s].nsb, R.nvb = hi[s].nvb;
         R.direct = hi[s].cmode == 1;
+        const unsigned long long tks[6] = {o.tk_asm, o.tk_lu, o.tk_solve, o.tk_mv, o.tk_other, o.tk_total};
+        for (int k = 0; k < 6; ++k) R.tk[k] = double(tks[k]) * 1e-8;
+        for (int k = 0; k < 4; ++k) R.tk[6 + k] = double(o.tk_lu_part[k]) * 1e-8;
         const long long N = (long long)hi[s].nsb * hi[s].nvb;
         if (opt.want_basis) {
             for (int ax = 0; ax < 2; ++ax) {

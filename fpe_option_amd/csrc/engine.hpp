@@ -14,6 +14,7 @@ struct SolveResult {
     int status = 0, n_acc = 0, n_rej = 0, n_lu = 0, n_newt = 0, n_err = 0, n_inner = 0, n_admm = 0;
     int nsb = 0, nvb = 0;
     bool direct = false;
+    double tk[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // seconds: assemble, lu, solve, mv, other, total, then factorisation parts: pivot inverse, staging, L multiply, Schur (solver workgroup)
     std::vector<double> q0, qT;              // filled when requested
     std::vector<std::vector<double>> hist;   // accepted states (stepwise API)
     std::vector<double> pdf;                 // n_qs x n_qv (stepwise API)

@@ -185,6 +185,15 @@ __global__ void __launch_bounds__(NT) k_radau(const Inst* __restrict__ inst, int
         }
         __syncthreads();
 
+        unsigned long long tk_asm = 0, tk_lu = 0, tk_solve = 0, tk_mv = 0;
+        if (tid == 0) S.lu.tk[0] = S.lu.tk[1] = S.lu.tk[2] = S.lu.tk[3] = 0;
+        const unsigned long long tk_start = wall_clock64();
+        unsigned long long tk_mark = tk_start;
+        auto lap = [&](unsigned long long& acc) {
+            const unsigned long long now = wall_clock64();
+            acc += now - tk_mark;
+            tk_mark = now;
+        };
         const double t0 = 0.0, t1 = in.T, posneg = 1.0, uround = 1e-16;
         const int nit = 6;
         const double safety = 0.9, fac1 = 0.2, fac2 = 10.0, quot1 = 1.0, quot2 = 1.2;
@@ -211,13 +220,18 @@ __global__ void __launch_bounds__(NT) k_radau(const Inst* __restrict__ inst, int
                 need_lu = ratio < quot1 || ratio > quot2;
             }
             if (need_lu) {
+                tk_mark = wall_clock64();
                 assemble_E<false>(Ms, Ks, ns, m, U1, 0.0, h, P, nullptr);
                 __syncthreads();
+                lap(tk_asm);
                 int bad = block_lu<false>(P, nullptr, ns, m, S.lu);
+                lap(tk_lu);
                 if (direct) {
                     assemble_E<true>(Ms, Ks, ns, m, ALPH, BETA, h, Cr, Ci);
                     __syncthreads();
+                    lap(tk_asm);
                     bad |= block_lu<true>(Cr, Ci, ns, m, S.lu);
+                    lap(tk_lu);
                 }
                 (void)bad;
                 h_lu = h;
@@ -250,8 +264,10 @@ __global__ void __launch_bounds__(NT) k_radau(const Inst* __restrict__ inst, int
             double theta_loc = fabs(theta), dynold = 0.0, thqold = 0.0;
             bool converged = false, newt_fail = false;
             while (newt < nit) {
+                tk_mark = wall_clock64();
                 newton_mv(Ms, Ks, ns, nv, vb);
                 __syncthreads();
+                lap(tk_mv);
                 for (int r = tid; r < N; r += NT) {
                     const double wv = vv(S_W)[r];
                     const double f1 = -wv - vv(S_KZ1)[r], f2 = -wv - vv(S_KZ2)[r], f3 = -wv - vv(S_KZ3)[r];
@@ -267,12 +283,14 @@ __global__ void __launch_bounds__(NT) k_radau(const Inst* __restrict__ inst, int
                     else vv(S_VRE0)[r] = re, vv(S_VIM0)[r] = im;
                 }
                 __syncthreads();
+                lap(tk_mv);
                 if (direct) {
                     double* xr1[1] = {vv(S_DF1)};
                     block_solve<false, 1>(P, nullptr, ns, m, xr1, nullptr, S.sT);
                     double* xr2[1] = {vv(S_DF2)};
                     double* xi2[1] = {vv(S_DF3)};
                     block_solve<true, 1>(Cr, Ci, ns, m, xr2, xi2, S.sT);
+                    lap(tk_solve);
                 } else {
                     // dF1 and the preconditioned GMRES right-hand side in one pass
                     double* xr3[3] = {vv(S_DF1), vv(S_VRE0), vv(S_VIM0)};
@@ -441,8 +459,10 @@ __global__ void __launch_bounds__(NT) k_radau(const Inst* __restrict__ inst, int
             for (int r = tid; r < N; r += NT) vv(S_ERR)[r] = vv(S_MC)[r] - h * vv(S_W)[r];
             __syncthreads();
             {
+                tk_mark = wall_clock64();
                 double* xe[1] = {vv(S_ERR)};
                 block_solve<false, 1>(P, nullptr, ns, m, xe, nullptr, S.sT);
+                lap(tk_solve);
             }
             ++n_err;
             double en = 0.0;
@@ -515,6 +535,11 @@ __global__ void __launch_bounds__(NT) k_radau(const Inst* __restrict__ inst, int
             o.status = status;
             o.n_acc = n_acc, o.n_rej = n_rej, o.n_lu = n_lu, o.n_newt = n_newt, o.n_err = n_err;
             o.n_inner = n_inner, o.n_hist = nh, o.t_end = t, o.h_last = h;
+            o.tk_asm = tk_asm, o.tk_lu = tk_lu, o.tk_solve = tk_solve, o.tk_mv = tk_mv;
+            o.tk_total = wall_clock64() - tk_start;
+            o.tk_other = o.tk_total - tk_asm - tk_lu - tk_solve - tk_mv;
+            o.tk_lu_part[0] = S.lu.tk[0], o.tk_lu_part[1] = S.lu.tk[1];
+            o.tk_lu_part[2] = S.lu.tk[2], o.tk_lu_part[3] = S.lu.tk[3];
         }
         __syncthreads();
     }

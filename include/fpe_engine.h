@@ -171,6 +171,11 @@ typedef struct FpeBatchStats {
      * 14 n_sb n_vb^2 (x4 complex), 49-point SpMV 98 N; bytes = factor panels read
      * by every solve (7 n_sb n_vb^2 doubles real, x2 complex) plus factor writes. */
     double alg_flops, alg_bytes;
+    /* Solver-workgroup seconds summed over solves, by phase: stage-matrix
+     * assembly, block factorisation, block solves, SpMV + Newton update, rest. */
+    double wg_s_assemble, wg_s_lu, wg_s_solve, wg_s_spmv, wg_s_other, wg_s_total;
+    /* Factorisation seconds split: pivot-block inverse, operand staging, L multiply, Schur update. */
+    double wg_s_lu_inverse, wg_s_lu_stage, wg_s_lu_lmul, wg_s_lu_schur;
 } FpeBatchStats;
 
 /** CSR matrix (row-major, sorted columns). */
