@@ -31,9 +31,11 @@ def _oracle_worker(args):
     return p[0], time.perf_counter() - t
 
 
-def cpu_baseline(cols, n_sample, procs):
-    """FP64 oracle (a restatement of the reference algorithm) on a stratified
-    sample of the same batch, one option per process."""
+def cpu_baseline_start(cols, n_sample, procs):
+    """Starts the FP64 CPU oracle (the C++ restatement of the reference
+    algorithm, one option per process, one thread each) on options spread
+    evenly over the same batch. Runs while the GPU is timed; collect with
+    cpu_baseline_finish."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
     oracle_lib.build()
@@ -46,12 +48,34 @@ def cpu_baseline(cols, n_sample, procs):
         kw = {k: (v[i].item() if isinstance(v, np.ndarray) else v) for k, v in sub.items()
               if k not in ("strike", "precision")}
         jobs.append((kw, float(sub["strike"][i])))
-    t = time.perf_counter()
-    with mp.get_context("spawn").Pool(procs) as pool:
-        res = pool.map(_oracle_worker, jobs, chunksize=1)
-    wall = time.perf_counter() - t
+    pool = mp.get_context("spawn").Pool(procs)
+    return pool, pool.map_async(_oracle_worker, jobs, chunksize=1), idx, time.perf_counter()
+
+
+def cpu_baseline_finish(handle, procs):
+    pool, fut, idx, t0 = handle
+    res = fut.get()
+    wall = time.perf_counter() - t0
+    pool.close()
+    pool.join()
     prices = np.array([r[0] for r in res])
-    return n_sample / wall, wall, prices, idx
+    cpu_s = float(sum(r[1] for r in res))
+    # aggregate throughput of `procs` cores kept busy: options per core-second x cores
+    return procs * len(res) / cpu_s, len(res) / cpu_s, cpu_s, wall, prices, idx
+
+
+def copy_calibration(mbytes):
+    """One plain device-to-device copy of a known size, for scaling the
+    FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
+    import torch
+    n = int(mbytes * (1 << 20)) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_each_way": 4 * n, "launches": 3}), flush=True)
 
 
 def main():
@@ -61,9 +85,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=4096, help="options per batch")
     ap.add_argument("--mode", default="direct", choices=["direct", "gmres"])
-    ap.add_argument("--cpu-sample", type=int, default=32)
+    ap.add_argument("--cpu-sample", type=int, default=15)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="MIB",
+                    help="only run a plain copy of this many MiB (counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy > 0:
+        copy_calibration(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU sharding is design-only",
                           "n_gpus": args.gpus}))
@@ -71,6 +100,9 @@ def main():
 
     from fpe_option_amd import api, workloads
     cols = workloads.batch_columns(args.n, complex_mode=1 if args.mode == "direct" else 0)
+    cores = len(os.sched_getaffinity(0))
+    procs = max(1, min(15, cores - 1))
+    cpu = None if args.no_cpu else cpu_baseline_start(cols, args.cpu_sample, procs)
     for _ in range(args.warmup):
         api.price_batch(**cols)
     t0 = time.perf_counter()
@@ -107,14 +139,15 @@ def main():
                      "hbm_frac": round(achieved_gbs / HBM_PEAK_GBS, 5)},
         "cpu_baseline": None,
     }
-    if not args.no_cpu:
-        cores = len(os.sched_getaffinity(0))
-        procs = max(1, min(16, cores))
-        rate, cwall, cprices, idx = cpu_baseline(cols, args.cpu_sample, procs)
+    if cpu is not None:
+        rate, rate1, cpu_s, cwall, cprices, idx = cpu_baseline_finish(cpu, procs)
         d = np.abs(prices[idx] - cprices)
         line["cpu_baseline"] = {"value": round(rate, 4), "unit": "options/s", "cores": procs, "kind": "port",
-                                "sample": f"{args.cpu_sample} options spread evenly over the same batch, "
-                                          f"one per process, {cwall:.1f} s wall",
+                                "sample": f"{len(idx)} options spread evenly over the same batch (all four grids), "
+                                          f"one single-threaded oracle process per option on {procs} host cores, "
+                                          f"run beside the GPU timing; value = cores x options / "
+                                          f"{cpu_s:.1f} core-seconds ({cwall:.1f} s wall)",
+                                "per_core": round(rate1, 5),
                                 "max_abs_dprice_vs_gpu": float(np.max(d))}
     print(json.dumps(line), flush=True)
 
