@@ -64,6 +64,21 @@ def cpu_baseline_finish(handle, procs):
     return procs * len(res) / cpu_s, len(res) / cpu_s, cpu_s, wall, prices, idx
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_radau_batch4096_hbm_counters.md")
+
+
+def profiled_traffic(n, mode):
+    """HBM bytes per k_radau launch from the committed FETCH_SIZE / WRITE_SIZE
+    counter runs of this same workload (4096 options, direct mode), scaled by
+    the copy-kernel calibration recorded there; None for other workloads."""
+    if n != 4096 or mode != "direct" or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    for line in open(TRAFFIC_PROFILE):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def copy_calibration(mbytes):
     """One plain device-to-device copy of a known size, for scaling the
     FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
@@ -134,7 +149,10 @@ def main():
                                            "n_err_solves", "n_inner_iters", "n_admm_iters")},
         "roofline": {"bound": "mfma", "kernel": "k_radau", "achieved": round(achieved_tf, 4),
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved_tf / FP64_PEAK_TFLOPS, 5),
-                     "traffic": None, "alg_bytes_per_launch": st["alg_bytes"],
+                     "traffic": profiled_traffic(args.n, args.mode),
+                     "traffic_source": "rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1 (copy-calibrated), "
+                                       "profiles/k_radau_batch4096_hbm_counters.md",
+                     "alg_bytes_per_launch": st["alg_bytes"],
                      "alg_flops_per_launch": st["alg_flops"], "hbm_achieved_GBs": round(achieved_gbs, 2),
                      "hbm_frac": round(achieved_gbs / HBM_PEAK_GBS, 5)},
         "cpu_baseline": None,
