@@ -121,3 +121,13 @@ def test_python_surface_validation_errors():
     with pytest.raises(ValueError):
         fe.Compute(S0=60.0, option_type="up_and_out_call", barrier=50.0)
     assert fe._OPTION_TYPES["european_put"] == 9
+
+
+def test_native_module_surface():
+    # the names the reference's Python layer imports from its native module
+    # (src/bindings/_fpe_native.mojo:40-50) and the methods pricer.Compute calls
+    from fpe_option_amd import native
+    assert callable(native.price)
+    for m in ("knots", "grid_points", "basis_1d", "basis_2d", "initial_condition", "solve", "pdf",
+              "payoff_price", "greeks"):
+        assert callable(getattr(native.Compute, m)), m

@@ -188,3 +188,21 @@ def test_invalid_options_get_status(engine):
                                      option_type=[8, 8, 2], barrier=[0.0, 0.0, 70.0])
     assert list(status) == [0, 1, 1]
     assert prices[1] == 0.0 and prices[2] == 0.0 and prices[0] > 0
+
+
+def test_native_module_drives_reference_shaped_wrapper(engine, oracle):
+    # the dict/array shapes the reference's pricer.Compute converts (pricer.py:124-197)
+    import scipy.sparse as sp
+    from fpe_option_amd import native
+    c = native.Compute(**HESTON, n_s=16, n_v=16, option_type=2, barrier=50.0, num_insert=50)
+    k = c.knots()
+    assert set(k) == {"s", "v"}
+    b = c.basis_1d()
+    Bs = sp.csr_matrix((b["Bs"]["data"], b["Bs"]["indices"], b["Bs"]["indptr"]),
+                       shape=(b["Bs"]["nrows"], b["Bs"]["ncols"]))
+    P = oracle.Pipe(**HESTON, n_s=16, n_v=16, num_insert=50, s_max=180.0, option_type=2, barrier=50.0)
+    np.testing.assert_allclose(Bs.toarray(), P.basis(0, 0).toarray(), atol=1e-14)
+    p = c.payoff_price([60.0, 70.0])
+    assert len(p) == 2 and np.all(np.isfinite(p))
+    r = native.price(dict(HESTON, K=[60.0], option_type=8, n_s=16, n_v=16))
+    assert set(r) == {"prices", "deltas", "gammas", "vegas"} and len(r["prices"]) == 1
