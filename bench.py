@@ -79,6 +79,47 @@ def profiled_traffic(n, mode):
     return None
 
 
+def surrogate_leg(plain_value, n=1_000_000, audit_frac=0.01, n_out=256, seed=20261205):
+    """The learned path on n seeded European options from the training box
+    (plus n_out rows outside it, which go to the solver): MLP + POD pricing on
+    the GPU, a random audit fraction re-solved by the plain FP64 solver on the
+    same grid, throughput over the whole job (inference + fallback + audit)."""
+    from fpe_option_amd import surrogate as S
+    if not os.path.exists(S.MODEL_PATH):
+        return None
+    sur = S.Surrogate()
+    rng = np.random.Generator(np.random.PCG64(seed))
+    P = S.sample_params(n, seed)
+    out_rows = rng.choice(n, n_out, replace=False)
+    P[out_rows, 2] = rng.uniform(0.85, 1.0, n_out)  # sigma above the box
+    K = rng.uniform(45.0, 75.0, n)
+    ty = np.where(rng.uniform(0.0, 1.0, n) < 0.5, 8, 9).astype(np.int32)
+    sur.price(P[:4096], K[:4096], ty[:4096])  # warm-up (weights resident, kernels loaded)
+    t0 = time.perf_counter()
+    prices, ok, n_fb = sur.price_with_fallback(P, K, ty)
+    t1 = time.perf_counter()
+    rows = np.nonzero(ok)[0]
+    pick = np.sort(rng.choice(rows, size=int(round(audit_frac * len(rows))), replace=False))
+    ref, st = S.solver_prices(P[pick], K[pick], ty[pick])
+    t2 = time.perf_counter()
+    d = np.abs(prices[pick] - ref)[st == 0]
+    value = n / (t2 - t0)
+    return {"options": n, "value": round(value, 1), "unit": "options/s",
+            "vs_plain_fp64": round(value / plain_value, 1) if plain_value else None,
+            "inference_options_per_s": round(int(ok.sum()) / max(1e-9, t1 - t0 - 0.0), 1),
+            "seconds": {"surrogate_and_fallback": round(t1 - t0, 3), "audit": round(t2 - t1, 3)},
+            "fallback_solves": int(n_fb),
+            "audit": {"n": int(len(pick)), "max_abs_dprice": float(np.max(d)) if len(d) else None,
+                      "p99_abs_dprice": float(np.percentile(d, 99)) if len(d) else None,
+                      "mean_abs_dprice": float(np.mean(d)) if len(d) else None,
+                      "judge": "plain FP64 solve, same 80x40 grid"},
+            "model": {"samples": int(sur.info["meta_samples"]), "val_max_abs_dprice":
+                      float(sur.info["meta_val_max_abs_dprice"])},
+            "workload": f"{n} seeded European calls/puts from the training box (kappa, theta, sigma, rho, V0, r, T), "
+                        f"K in [45, 75], S0 = 60, {n_out} of them outside the box (solver fallback); "
+                        f"{audit_frac:.0%} audit"}
+
+
 def copy_calibration(mbytes):
     """One plain device-to-device copy of a known size, for scaling the
     FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
@@ -102,6 +143,8 @@ def main():
     ap.add_argument("--mode", default="direct", choices=["direct", "gmres"])
     ap.add_argument("--cpu-sample", type=int, default=15)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="MIB",
                     help="only run a plain copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -157,6 +200,24 @@ def main():
                      "hbm_frac": round(achieved_gbs / HBM_PEAK_GBS, 5)},
         "cpu_baseline": None,
     }
+    if not args.no_mixed:
+        mcols = dict(cols, precision=1, complex_mode=1)
+        t = time.perf_counter()
+        mprices, mstatus, mst = api.price_batch(return_stats=True, **mcols)
+        mwall = time.perf_counter() - t
+        ok = (mstatus == 0) & (status == 0)
+        d = np.abs(mprices - prices)[ok]
+        line["mixed"] = {"value": round(args.n / mwall, 3), "unit": "options/s", "ms_per_step": round(1e3 * mwall, 1),
+                         "vs_plain_fp64": round((args.n / mwall) / value, 3),
+                         "path": "FP32 block factorisation on v_mfma_f32_16x16x4f32, FP64 refinement of every "
+                                 "stage solve to |r| <= 1e-12 |b|; FP64 Newton residuals, error control, payoff",
+                         "max_abs_dprice_vs_fp64": float(np.max(d)), "p99_abs_dprice_vs_fp64": float(np.percentile(d, 99)),
+                         "refine_rounds_per_stage_solve": round(mst["n_refine"] / max(1, 2 * mst["n_newton"] +
+                                                                                       mst["n_err_solves"]), 3),
+                         "phases_ms": {k: round(mst[k], 1) for k in ("ms_ic", "ms_radau")},
+                         "status_ok": int(np.sum(mstatus == 0))}
+    if not args.no_surrogate:
+        line["mixed_surrogate"] = surrogate_leg(value)
     if cpu is not None:
         rate, rate1, cpu_s, cwall, cprices, idx = cpu_baseline_finish(cpu, procs)
         d = np.abs(prices[idx] - cprices)

@@ -57,7 +57,8 @@ class FpeBatchOption(C.Structure):
                                            "s_min", "s_max", "v_max", "barrier", "strike")] + [
         (n, C.c_int32) for n in ("option_type", "n_s", "n_v", "disc_mode", "num_insert",
                                  "complex_mode", "precision", "exact_constants")] + [
-        (n, C.c_double) for n in ("ic_sigma_s", "ic_sigma_v", "rtol", "atol", "knot_width_s", "knot_width_v")]
+        (n, C.c_double) for n in ("ic_sigma_s", "ic_sigma_v", "rtol", "atol", "knot_width_s", "knot_width_v",
+                                  "knot_center_v")]
 
 
 class FpeBatchStats(C.Structure):
@@ -66,7 +67,7 @@ class FpeBatchStats(C.Structure):
         (n, C.c_double) for n in ("ms_setup", "ms_assemble", "ms_ic", "ms_radau", "ms_payoff", "ms_total",
                                   "alg_flops", "alg_bytes", "wg_s_assemble", "wg_s_lu", "wg_s_solve",
                                   "wg_s_spmv", "wg_s_other", "wg_s_total", "wg_s_lu_inverse", "wg_s_lu_stage",
-                                  "wg_s_lu_lmul", "wg_s_lu_schur")]
+                                  "wg_s_lu_lmul", "wg_s_lu_schur")] + [("n_refine", C.c_int64)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -96,6 +97,22 @@ SIGNATURES = {
     "fpe_compute_free_csr": ([C.POINTER(FpeCsrResult)], None),
     "fpe_batch_option_default": ([C.POINTER(FpeBatchOption)], None),
     "fpe_batch_price": ([C.POINTER(FpeBatchOption), _I, _DP, C.POINTER(C.c_int32), C.POINTER(FpeBatchStats)], _I),
+    "fpe_batch_terminal_states": ([C.POINTER(FpeBatchOption), _I, _DP, C.c_int64, C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_int32), C.POINTER(FpeBatchStats)], _I),
+    "fpe_european_pricing_operator": ([C.POINTER(FpeBatchOption), _I, _I, _DP, _DP, C.POINTER(C.c_int32),
+                                       C.POINTER(C.c_int32)], _I),
+    "fpe_mlp_forward": ([_P, _I, _I, _P, _P, _I, _I, _P, _P, _P], _I),
+    "fpe_mlp_backward": ([_P, _P, _P, _I, _I, _I, _P, _P, _P, _P, _P], _I),
+    "fpe_mlp_adam": ([_P, _P, _P, _P, _P, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float, _I, _P], _I),
+    "fpe_mlp_mse": ([_P, _P, _P, _I, _I, _P, _P, _P], _I),
+    "fpe_mlp_to_bf16": ([_P, _P, C.c_int64, _P], _I),
+    "fpe_mlp_gather_rows": ([_P, _P, _I, _I, _P, _P], _I),
+    "fpe_device_alloc": ([C.c_int64], _P),
+    "fpe_device_free": ([_P], None),
+    "fpe_device_upload": ([_P, _P, C.c_int64], _I),
+    "fpe_device_download": ([_P, _P, C.c_int64], _I),
+    "fpe_device_synchronize": ([], _I),
+    "fpe_surrogate_price": ([_P, _I, _I, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P], _I),
     "fpe_last_error": ([], C.c_char_p),
     "fpe_device_count": ([], _I),
 }

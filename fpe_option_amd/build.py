@@ -36,11 +36,12 @@ def build(verbose=False):
         o = os.path.join(OUT_DIR, src + ".o")
         _run([CXX] + CXX_FLAGS + ["-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
-    o = os.path.join(OUT_DIR, "engine.hip.o")
-    msg = _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "engine.hip"), "-o", o])
-    if verbose and msg:
-        print(msg)
-    objs.append(o)
+    for src in ("engine.hip", "surrogate.hip"):
+        o = os.path.join(OUT_DIR, src + ".o")
+        msg = _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", o])
+        if verbose and msg:
+            print(msg)
+        objs.append(o)
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs)
     os.replace(tmp, LIB)

@@ -40,12 +40,33 @@ struct LuLds {
 // probe): a = A[row0 + lane%16][k0 + lane/16], b = B[k0 + lane/16][col0 + lane%16],
 // acc[j] = C[row0 + lane/16 + 4*j][col0 + lane%16]. (The f32 16x16x4 form places
 // acc[j] at row 4*(lane/16) + j instead.)
-__device__ __forceinline__ void mfma_c(d4& accr, d4& acci, double ar, double ai, double br, double bi, bool cplx) {
-    accr = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, accr, 0, 0, 0);
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// Scalar-type traits of the factorisation: FP64 uses v_mfma_f64_16x16x4f64,
+// the mixed-precision path FP32 panels with v_mfma_f32_16x16x4f32.
+template <class T> struct Mf;
+template <> struct Mf<double> {
+    typedef d4 acc;
+    static __device__ __forceinline__ d4 mfma(double a, double b, d4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int j) { return (lane >> 4) + 4 * j; }
+};
+template <> struct Mf<float> {
+    typedef f4 acc;
+    static __device__ __forceinline__ f4 mfma(float a, float b, f4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int j) { return 4 * (lane >> 4) + j; }
+};
+
+template <class T, class A>
+__device__ __forceinline__ void mfma_c(A& accr, A& acci, T ar, T ai, T br, T bi, bool cplx) {
+    accr = Mf<T>::mfma(ar, br, accr);
     if (cplx) {
-        accr = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bi, accr, 0, 0, 0);
-        acci = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, acci, 0, 0, 0);
-        acci = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, acci, 0, 0, 0);
+        accr = Mf<T>::mfma(-ai, bi, accr);
+        acci = Mf<T>::mfma(ar, bi, acci);
+        acci = Mf<T>::mfma(ai, br, acci);
     }
 }
 
@@ -54,6 +75,15 @@ __device__ __forceinline__ double readlane_d(double x, int l) {
     const unsigned lo = (unsigned)__builtin_amdgcn_readlane(int(unsigned(v)), l);
     const unsigned hi = (unsigned)__builtin_amdgcn_readlane(int(unsigned(v >> 32)), l);
     return __builtin_bit_cast(double, (unsigned long long)lo | ((unsigned long long)hi << 32));
+}
+
+__device__ __forceinline__ float readlane_d(float x, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), l));
+}
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_d(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, 0xF, 0xF, false));
 }
 
 template <int CTRL>
@@ -66,13 +96,35 @@ __device__ __forceinline__ double dpp_d(double x) {
 
 // Wave-uniform maximum of a double: DPP row rotations (8, 4, 2, 1) leave each
 // 16-lane row's maximum in all its lanes, then the four rows are combined.
-__device__ __forceinline__ double wave_max_dpp(double v) {
+template <class T>
+__device__ __forceinline__ T wave_max_dpp(T v) {
     v = fmax(v, dpp_d<0x128>(v));
     v = fmax(v, dpp_d<0x124>(v));
     v = fmax(v, dpp_d<0x122>(v));
     v = fmax(v, dpp_d<0x121>(v));
     return fmax(fmax(readlane_d(v, 0), readlane_d(v, 16)), fmax(readlane_d(v, 32), readlane_d(v, 48)));
 }
+
+// Pivot-search key: |value| with its 6 lowest significand bits replaced by
+// 63 - lane, so one max reduction yields the pivot row (ties to the lower row).
+__device__ __forceinline__ double pivot_key(double mag, int lane) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, mag);
+    return __builtin_bit_cast(double, (b & ~63ull) | (unsigned long long)(63 - lane));
+}
+__device__ __forceinline__ float pivot_key(float mag, int lane) {
+    const unsigned b = __builtin_bit_cast(unsigned, mag);
+    return __builtin_bit_cast(float, (b & ~63u) | (unsigned)(63 - lane));
+}
+__device__ __forceinline__ int key_lane(double key) {
+    return 63 - int(__builtin_bit_cast(unsigned long long, key) & 63ull);
+}
+__device__ __forceinline__ int key_lane(float key) { return 63 - int(__builtin_bit_cast(unsigned, key) & 63u); }
+
+// LDS staging block k viewed as elements of T (row stride LS).
+template <class T>
+__device__ __forceinline__ T* lds_block(LuLds& L, int k) { return reinterpret_cast<T*>(L.B[k]); }
+template <class T>
+__device__ __forceinline__ const T* lds_block(const LuLds& L, int k) { return reinterpret_cast<const T*>(L.B[k]); }
 
 // Gauss-Jordan inverse of the m x m block in (B[0], B[1]) by all four waves.
 // Lane i holds row i and wave w holds the columns j = w (mod 4) in registers,
@@ -87,36 +139,39 @@ __device__ __forceinline__ double wave_max_dpp(double v) {
 // X[p_k][j] = inv(A)[k][p_j]; the last pass unscrambles inv(A)[a][b] = X[p_a][q_b]
 // with p_{q_b} = b. Result in natural order in (B[0], B[1]); B[2], B[3] are
 // scratch. Sets singular on a zero or non-finite pivot. Ends with a barrier.
-template <bool CPLX>
+template <bool CPLX, class T>
 __device__ void gj_inverse(LuLds& L, int m) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const bool lrow = lane < m;
-    double xr[NRG], xi[NRG];
+    T* B0 = lds_block<T>(L, 0);
+    T* B1 = lds_block<T>(L, 1);
+    T* B2 = lds_block<T>(L, 2);
+    T* B3 = lds_block<T>(L, 3);
+    T xr[NRG], xi[NRG];
 #pragma unroll
     for (int r = 0; r < NRG; ++r) {
         const int j = NWAVE * r + w;
-        xr[r] = (lrow && j < m) ? L.B[0][lane * LS + j] : 0.0;
-        xi[r] = (CPLX && lrow && j < m) ? L.B[1][lane * LS + j] : 0.0;
+        xr[r] = (lrow && j < m) ? B0[lane * LS + j] : T(0);
+        xi[r] = (CPLX && lrow && j < m) ? B1[lane * LS + j] : T(0);
     }
     bool used = !lrow;
     if (threadIdx.x < MAX_BLOCK) L.qinv[threadIdx.x] = 0;  // stays in range if a bad pivot search repeats a row
 
     // pivot search of column k (owner wave): publishes the column, the pivot row and 1/pivot in buffer b
-    auto search = [&](int k, double cr, double ci, int b) {
-        const double mag = CPLX ? fabs(cr) + fabs(ci) : fabs(cr);
-        const unsigned long long bits = __builtin_bit_cast(unsigned long long, mag);
-        double key = used ? -1.0 : __builtin_bit_cast(double, (bits & ~63ull) | (unsigned long long)(63 - lane));
+    auto search = [&](int k, T cr, T ci, int b) {
+        const T mag = CPLX ? fabs(cr) + fabs(ci) : fabs(cr);
+        T key = used ? T(-1) : pivot_key(mag, lane);
         key = wave_max_dpp(key);
-        const int p = __builtin_amdgcn_readfirstlane(63 - int(__builtin_bit_cast(unsigned long long, key) & 63ull));
-        const double dr = readlane_d(cr, p), di = CPLX ? readlane_d(ci, p) : 0.0;
-        double ir, ii;
+        const int p = __builtin_amdgcn_readfirstlane(key_lane(key));
+        const T dr = readlane_d(cr, p), di = CPLX ? readlane_d(ci, p) : T(0);
+        T ir, ii;
         if (CPLX) {
-            const double den = dr * dr + di * di;
-            ir = den > 0.0 ? dr / den : 0.0;
-            ii = den > 0.0 ? -di / den : 0.0;
+            const T den = dr * dr + di * di;
+            ir = den > T(0) ? dr / den : T(0);
+            ii = den > T(0) ? -di / den : T(0);
         } else {
-            ir = dr != 0.0 ? 1.0 / dr : 0.0;
-            ii = 0.0;
+            ir = dr != T(0) ? T(1) / dr : T(0);
+            ii = T(0);
         }
         L.colr[b][lane] = cr;
         if (CPLX) L.coli[b][lane] = ci;
@@ -125,8 +180,8 @@ __device__ void gj_inverse(LuLds& L, int m) {
             L.ipr[b] = ir;
             L.ipi[b] = ii;
             L.piv[k] = p;
-            const double mp = CPLX ? fabs(dr) + fabs(di) : fabs(dr);
-            if (!(key > 0.0) || !(mp > 0.0) || !(mp < INFINITY)) L.singular = 1;
+            const T mp = CPLX ? fabs(dr) + fabs(di) : fabs(dr);
+            if (!(key > T(0)) || !(mp > T(0)) || !(mp < T(INFINITY))) L.singular = 1;
         }
     };
     if (w == 0) search(0, xr[0], xi[0], 0);
@@ -134,27 +189,27 @@ __device__ void gj_inverse(LuLds& L, int m) {
         const int b = k & 1;
         __syncthreads();
         const int p = __builtin_amdgcn_readfirstlane(L.pk[b]);
-        const double fr = L.colr[b][lane], fi = CPLX ? L.coli[b][lane] : 0.0;
-        const double ir = L.ipr[b], ii = CPLX ? L.ipi[b] : 0.0;
+        const T fr = T(L.colr[b][lane]), fi = CPLX ? T(L.coli[b][lane]) : T(0);
+        const T ir = T(L.ipr[b]), ii = CPLX ? T(L.ipi[b]) : T(0);
         const bool isp = lane == p;
         if (isp) used = true;
         // in-place step on the entry (row lane, column j): pivot row -> row / pivot,
         // column k -> -f / pivot (1 / pivot on the pivot row), rest -> x - f * row_p / pivot
-        auto upd = [&](double& vr, double& vi, int j) {
-            double sr, si;
+        auto upd = [&](T& vr, T& vi, int j) {
+            T sr, si;
             if (j == k) {
                 sr = ir, si = ii;
             } else {
-                const double pr = readlane_d(vr, p);
-                const double pi_ = CPLX ? readlane_d(vi, p) : 0.0;
+                const T pr = readlane_d(vr, p);
+                const T pi_ = CPLX ? readlane_d(vi, p) : T(0);
                 sr = CPLX ? pr * ir - pi_ * ii : pr * ir;
-                si = CPLX ? pr * ii + pi_ * ir : 0.0;
+                si = CPLX ? pr * ii + pi_ * ir : T(0);
             }
             if (isp) {
                 vr = sr;
                 if (CPLX) vi = si;
             } else {
-                const double br = (j == k) ? 0.0 : vr, bi = (j == k) ? 0.0 : vi;
+                const T br = (j == k) ? T(0) : vr, bi = (j == k) ? T(0) : vi;
                 vr = br - (CPLX ? fr * sr - fi * si : fr * sr);
                 if (CPLX) vi = bi - (fr * si + fi * sr);
             }
@@ -162,7 +217,7 @@ __device__ void gj_inverse(LuLds& L, int m) {
         const int k1 = k + 1;
         if (k1 < m && w == (k1 & (NWAVE - 1))) {
             const int r1 = k1 / NWAVE;
-            double cr = 0.0, ci = 0.0;
+            T cr = T(0), ci = T(0);
 #pragma unroll
             for (int r = 0; r < NRG; ++r)
                 if (r == r1) cr = xr[r], ci = xi[r];
@@ -179,8 +234,8 @@ __device__ void gj_inverse(LuLds& L, int m) {
     for (int r = 0; r < NRG; ++r) {
         const int j = NWAVE * r + w;
         if (lrow && j < m) {
-            L.B[2][lane * LS + j] = xr[r];
-            if (CPLX) L.B[3][lane * LS + j] = xi[r];
+            B2[lane * LS + j] = xr[r];
+            if (CPLX) B3[lane * LS + j] = xi[r];
         }
     }
     __syncthreads();
@@ -189,8 +244,8 @@ __device__ void gj_inverse(LuLds& L, int m) {
     for (int e = threadIdx.x; e < m * m; e += NT) {
         const int a = e / m, c = e - a * m;
         const int src = L.piv[a] * LS + L.qinv[c];
-        L.B[0][a * LS + c] = L.B[2][src];
-        if (CPLX) L.B[1][a * LS + c] = L.B[3][src];
+        B0[a * LS + c] = B2[src];
+        if (CPLX) B1[a * LS + c] = B3[src];
     }
     __syncthreads();
 }
@@ -206,9 +261,12 @@ __device__ __forceinline__ long long stacked_row(long long ps, int ld, int I, in
 // L_JI = A_JI Ainv_I for J = I+1..I+nU, in place. A rows come straight from the
 // panels (a wave takes 16-row strips; their fragments stay in registers), the
 // inverse from LDS (B[0], B[1]).
-template <bool CPLX>
-__device__ void lower_multiply(double* __restrict__ Pr, double* __restrict__ Pi, long long ps, int ld, int I, int m,
-                               int nU, const LuLds& L) {
+template <bool CPLX, class T>
+__device__ void lower_multiply(T* __restrict__ Pr, T* __restrict__ Pi, long long ps, int ld, int I, int m, int nU,
+                               const LuLds& L) {
+    typedef typename Mf<T>::acc A4;
+    const T* B0 = lds_block<T>(L, 0);
+    const T* B1 = lds_block<T>(L, 1);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, l4 = lane >> 4;
     const int M = nU * m, tm = (M + 15) >> 4, tk = (m + 3) >> 2, tn = (m + 15) >> 4;
@@ -217,19 +275,19 @@ __device__ void lower_multiply(double* __restrict__ Pr, double* __restrict__ Pi,
         const bool va = ra < M;
         int dJa;
         const long long oa = va ? stacked_row(ps, ld, I, m, ra, dJa) + (3 - dJa) * m : 0;
-        double ar[MAX_BLOCK / 4], ai[MAX_BLOCK / 4];
+        T ar[MAX_BLOCK / 4], ai[MAX_BLOCK / 4];
 #pragma unroll
         for (int s = 0; s < MAX_BLOCK / 4; ++s) {
             const int k = 4 * s + l4;
             const bool ok = s < tk && va && k < m;
-            ar[s] = ok ? Pr[oa + k] : 0.0;
-            ai[s] = (CPLX && ok) ? Pi[oa + k] : 0.0;
+            ar[s] = ok ? Pr[oa + k] : T(0);
+            ai[s] = (CPLX && ok) ? Pi[oa + k] : T(0);
         }
         long long oc[4];
         bool vc[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const int r = r0 + l4 + 4 * j;
+            const int r = r0 + Mf<T>::row(lane, j);
             vc[j] = r < M;
             int dJ;
             oc[j] = vc[j] ? stacked_row(ps, ld, I, m, r, dJ) + (3 - dJ) * m : 0;
@@ -237,14 +295,14 @@ __device__ void lower_multiply(double* __restrict__ Pr, double* __restrict__ Pi,
         for (int u = 0; u < tn; ++u) {
             const int c = (u << 4) + l16;
             const bool cv = c < m;
-            d4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
+            A4 accr = {0, 0, 0, 0}, acci = {0, 0, 0, 0};
 #pragma unroll
             for (int s = 0; s < MAX_BLOCK / 4; ++s) {
                 if (s < tk) {
                     const int k = 4 * s + l4;
                     const bool ok = cv && k < m;
-                    const double br = ok ? L.B[0][k * LS + c] : 0.0;
-                    const double bi = (CPLX && ok) ? L.B[1][k * LS + c] : 0.0;
+                    const T br = ok ? B0[k * LS + c] : T(0);
+                    const T bi = (CPLX && ok) ? B1[k * LS + c] : T(0);
                     mfma_c(accr, acci, ar[s], ai[s], br, bi, CPLX);
                 }
             }
@@ -264,9 +322,10 @@ __device__ void lower_multiply(double* __restrict__ Pr, double* __restrict__ Pi,
 // staged in LDS (B[2 kb] re, B[2 kb + 1] im in complex arithmetic; B[kb] in real).
 // L strips come from the panels into registers once per strip; the C tile of
 // the next MFMA chain is loaded before the chain so its latency overlaps it.
-template <bool CPLX>
-__device__ void schur_pass(double* __restrict__ Pr, double* __restrict__ Pi, long long ps, int ld, int I, int m,
-                           int nU, int kb0, int nkb, const LuLds& L) {
+template <bool CPLX, class T>
+__device__ void schur_pass(T* __restrict__ Pr, T* __restrict__ Pi, long long ps, int ld, int I, int m, int nU,
+                           int kb0, int nkb, const LuLds& L) {
+    typedef typename Mf<T>::acc A4;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, l4 = lane >> 4;
     const int M = nU * m, tm = (M + 15) >> 4, tk = (m + 3) >> 2, tn = (m + 15) >> 4;
@@ -275,48 +334,48 @@ __device__ void schur_pass(double* __restrict__ Pr, double* __restrict__ Pi, lon
         const bool va = ra < M;
         int dJa;
         const long long oa = va ? stacked_row(ps, ld, I, m, ra, dJa) + (3 - dJa) * m : 0;
-        double ar[MAX_BLOCK / 4], ai[MAX_BLOCK / 4];
+        T ar[MAX_BLOCK / 4], ai[MAX_BLOCK / 4];
 #pragma unroll
         for (int s = 0; s < MAX_BLOCK / 4; ++s) {
             const int k = 4 * s + l4;
             const bool ok = s < tk && va && k < m;
-            ar[s] = ok ? Pr[oa + k] : 0.0;
-            ai[s] = (CPLX && ok) ? Pi[oa + k] : 0.0;
+            ar[s] = ok ? Pr[oa + k] : T(0);
+            ai[s] = (CPLX && ok) ? Pi[oa + k] : T(0);
         }
         long long oc[4];
         int dj[4];
         bool vc[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const int r = r0 + l4 + 4 * j;
+            const int r = r0 + Mf<T>::row(lane, j);
             vc[j] = r < M;
             dj[j] = 1;
             oc[j] = vc[j] ? stacked_row(ps, ld, I, m, r, dj[j]) : 0;
         }
         for (int kb = 0; kb < nkb; ++kb) {
-            const double* Br = CPLX ? L.B[2 * kb] : L.B[kb];
-            const double* Bi = CPLX ? L.B[2 * kb + 1] : L.B[kb];
+            const T* Br = lds_block<T>(L, CPLX ? 2 * kb : kb);
+            const T* Bi = lds_block<T>(L, CPLX ? 2 * kb + 1 : kb);
             const int kk = kb0 + kb;  // K = I + 1 + kk; column block d = kk + 4 - dJ in panel I + dJ
             for (int u = 0; u < tn; ++u) {
                 const int c = (u << 4) + l16;
                 const bool cv = c < m;
-                double cr[4], ci[4];
+                T cr[4], ci[4];
                 long long o[4];
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     o[j] = oc[j] + (long long)(kk + 4 - dj[j]) * m + c;
                     const bool ok = vc[j] && cv;
-                    cr[j] = ok ? Pr[o[j]] : 0.0;
-                    ci[j] = (CPLX && ok) ? Pi[o[j]] : 0.0;
+                    cr[j] = ok ? Pr[o[j]] : T(0);
+                    ci[j] = (CPLX && ok) ? Pi[o[j]] : T(0);
                 }
-                d4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
+                A4 accr = {0, 0, 0, 0}, acci = {0, 0, 0, 0};
 #pragma unroll
                 for (int s = 0; s < MAX_BLOCK / 4; ++s) {
                     if (s < tk) {
                         const int k = 4 * s + l4;
                         const bool ok = cv && k < m;
-                        const double br = ok ? Br[k * LS + c] : 0.0;
-                        const double bi = (CPLX && ok) ? Bi[k * LS + c] : 0.0;
+                        const T br = ok ? Br[k * LS + c] : T(0);
+                        const T bi = (CPLX && ok) ? Bi[k * LS + c] : T(0);
                         mfma_c(accr, acci, ar[s], ai[s], br, bi, CPLX);
                     }
                 }
@@ -335,25 +394,27 @@ __device__ void schur_pass(double* __restrict__ Pr, double* __restrict__ Pi, lon
 // stencils: a coalesced 16-byte zero fill of the panels, then one thread per
 // row scatters its (at most) 49 stencil entries, whose loads are coalesced
 // across threads (stencil planes are row-contiguous). Panels are 16-byte aligned.
-template <bool CPLX>
+template <bool CPLX, class T>
 __device__ void assemble_E(const double* __restrict__ Ms, const double* __restrict__ Ks, int ns, int m,
-                           double ar, double ai, double h, double* __restrict__ Pr, double* __restrict__ Pi) {
+                           double ar, double ai, double h, T* __restrict__ Pr, T* __restrict__ Pi) {
     const int N = ns * m, ld = 7 * m;
-    const long long tot = (long long)N * ld, half = tot >> 1;
+    const long long tot = (long long)N * ld;
+    constexpr int V = int(16 / sizeof(T));  // elements per 16-byte store
+    const long long nvec = tot / V;
     const double2 z = {0.0, 0.0};
-    for (long long e = threadIdx.x; e < half; e += NT) {
+    for (long long e = threadIdx.x; e < nvec; e += NT) {
         reinterpret_cast<double2*>(Pr)[e] = z;
         if (CPLX) reinterpret_cast<double2*>(Pi)[e] = z;
     }
-    if ((tot & 1) && threadIdx.x == 0) {
-        Pr[tot - 1] = 0.0;
-        if (CPLX) Pi[tot - 1] = 0.0;
+    for (long long e = nvec * V + threadIdx.x; e < tot; e += NT) {
+        Pr[e] = T(0);
+        if (CPLX) Pi[e] = T(0);
     }
     __syncthreads();
     for (int row = threadIdx.x; row < N; row += NT) {
         const int I = row / m, i = row - I * m;
-        double* pr = Pr + (long long)row * ld;
-        double* pi = CPLX ? Pi + (long long)row * ld : nullptr;
+        T* pr = Pr + (long long)row * ld;
+        T* pi = CPLX ? Pi + (long long)row * ld : nullptr;
         for (int d = 0; d < 7; ++d) {
             const int J = I + d - 3;
             if (J < 0 || J >= ns) continue;
@@ -370,8 +431,8 @@ __device__ void assemble_E(const double* __restrict__ Ms, const double* __restri
             for (int q = 0; q < 7; ++q) {
                 const int j = i + q - 3;
                 if (j >= 0 && j < m) {
-                    pr[d * m + j] = ar * mv[q] + h * kv[q];
-                    if (CPLX) pi[d * m + j] = ai * mv[q];
+                    pr[d * m + j] = T(ar * mv[q] + h * kv[q]);
+                    if (CPLX) pi[d * m + j] = T(ai * mv[q]);
                 }
             }
         }
@@ -379,8 +440,8 @@ __device__ void assemble_E(const double* __restrict__ Ms, const double* __restri
 }
 
 // Factor the panels in place. Returns nonzero if a diagonal block was singular.
-template <bool CPLX>
-__device__ __noinline__ int block_lu(double* __restrict__ Pr, double* __restrict__ Pi, int ns, int m, LuLds& L) {
+template <bool CPLX, class T>
+__device__ __noinline__ int block_lu(T* __restrict__ Pr, T* __restrict__ Pi, int ns, int m, LuLds& L) {
     const int ld = 7 * m;
     const long long ps = (long long)m * ld;
     const int tid = threadIdx.x;
@@ -395,33 +456,35 @@ __device__ __noinline__ int block_lu(double* __restrict__ Pr, double* __restrict
     };
     // blocks per Schur pass: 3 U blocks fit the staging buffers in real arithmetic, 2 in complex
     constexpr int KPASS = CPLX ? 2 : 3;
+    T* B0 = lds_block<T>(L, 0);
+    T* B1 = lds_block<T>(L, 1);
     for (int I = 0; I < ns; ++I) {
-        double* PIr = Pr + I * ps;
-        double* PIi = Pi + I * ps;
+        T* PIr = Pr + I * ps;
+        T* PIi = Pi + I * ps;
         for (int e = tid; e < m * m; e += NT) {
             const int i = e / m, j = e - i * m;
-            L.B[0][i * LS + j] = PIr[(long long)i * ld + 3 * m + j];
-            if (CPLX) L.B[1][i * LS + j] = PIi[(long long)i * ld + 3 * m + j];
+            B0[i * LS + j] = PIr[(long long)i * ld + 3 * m + j];
+            if (CPLX) B1[i * LS + j] = PIi[(long long)i * ld + 3 * m + j];
         }
         __syncthreads();
         lap(1);
-        gj_inverse<CPLX>(L, m);
+        gj_inverse<CPLX, T>(L, m);
         lap(0);
         for (int e = tid; e < m * m; e += NT) {
             const int i = e / m, j = e - i * m;
-            PIr[(long long)i * ld + 3 * m + j] = L.B[0][i * LS + j];
-            if (CPLX) PIi[(long long)i * ld + 3 * m + j] = L.B[1][i * LS + j];
+            PIr[(long long)i * ld + 3 * m + j] = B0[i * LS + j];
+            if (CPLX) PIi[(long long)i * ld + 3 * m + j] = B1[i * LS + j];
         }
         const int nU = (ns - 1 - I) < 3 ? (ns - 1 - I) : 3;
         if (nU == 0) break;
-        lower_multiply<CPLX>(Pr, Pi, ps, ld, I, m, nU, L);
+        lower_multiply<CPLX, T>(Pr, Pi, ps, ld, I, m, nU, L);
         __syncthreads();
         lap(2);
         for (int kb0 = 0; kb0 < nU; kb0 += KPASS) {
             const int nkb = (nU - kb0) < KPASS ? (nU - kb0) : KPASS;
             for (int kb = 0; kb < nkb; ++kb) {
-                double* Br = CPLX ? L.B[2 * kb] : L.B[kb];
-                double* Bi = CPLX ? L.B[2 * kb + 1] : nullptr;
+                T* Br = lds_block<T>(L, CPLX ? 2 * kb : kb);
+                T* Bi = CPLX ? lds_block<T>(L, 2 * kb + 1) : nullptr;
                 const int col = (4 + kb0 + kb) * m;
                 for (int e = tid; e < m * m; e += NT) {
                     const int i = e / m, j = e - i * m;
@@ -431,7 +494,7 @@ __device__ __noinline__ int block_lu(double* __restrict__ Pr, double* __restrict
             }
             __syncthreads();
             lap(1);
-            schur_pass<CPLX>(Pr, Pi, ps, ld, I, m, nU, kb0, nkb, L);
+            schur_pass<CPLX, T>(Pr, Pi, ps, ld, I, m, nU, kb0, nkb, L);
             __syncthreads();
             lap(3);
         }
@@ -440,11 +503,127 @@ __device__ __noinline__ int block_lu(double* __restrict__ Pr, double* __restrict
     return L.singular;
 }
 
+// Single right-hand side solve with the block factors, latency-shaped: each
+// thread (row i = tid / 4, part q = tid % 4) loads a block-row's factor
+// entries block by block (16 per thread, unrolled and predicated), issuing the
+// next block's loads before contracting the current one, so a block-row step
+// waits for about one memory latency; the last solution blocks live in an LDS
+// ring (4 x 64, re and im) instead of being re-read from memory. ring: LDS of
+// at least 10 * 64 doubles. Factors of type T (FP64, or FP32 for the
+// mixed-precision path) are widened to FP64; all arithmetic is FP64.
+template <bool CPLX, class T>
+__device__ __noinline__ void block_solve1(const T* __restrict__ Pr, const T* __restrict__ Pi, int ns, int m,
+                                          double* __restrict__ xr, double* __restrict__ xi, double* ring) {
+    const int ld = 7 * m;
+    const long long ps = (long long)m * ld;
+    const int tid = threadIdx.x, i = tid >> 2, q = tid & 3;
+    const bool row = i < m;
+    double* rr = ring;
+    double* ri = ring + 4 * 64;
+    double* tr = ring + 8 * 64;
+    double* ti = ring + 9 * 64;
+    constexpr int U = MAX_BLOCK / 4;
+    double a0[U], b0[U], a1[U], b1[U];
+    const T* pr = nullptr;
+    const T* pi = nullptr;
+    // factor entries (row i, columns col + q + 4u) of the current block row
+    auto load = [&](int col, bool ok, double* a, double* b) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = q + 4 * u;
+            const bool v = ok && row && k < m;
+            a[u] = v ? double(pr[col + k]) : 0.0;
+            b[u] = (CPLX && v) ? double(pi[col + k]) : 0.0;
+        }
+    };
+    // (ar, ai) += block row segment (a, b) times the vector block (yr, yi)
+    auto contract = [&](const double* a, const double* b, const double* yr, const double* yi, double& ar,
+                        double& ai) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = q + 4 * u;
+            if (k < m) {
+                if (CPLX) {
+                    ar += a[u] * yr[k] - b[u] * yi[k];
+                    ai += a[u] * yi[k] + b[u] * yr[k];
+                } else {
+                    ar += a[u] * yr[k];
+                }
+            }
+        }
+    };
+    auto reduce4 = [&](double& ar, double& ai) {
+        ar += __shfl_xor(ar, 1, 64);
+        ar += __shfl_xor(ar, 2, 64);
+        if (CPLX) {
+            ai += __shfl_xor(ai, 1, 64);
+            ai += __shfl_xor(ai, 2, 64);
+        }
+    };
+    // forward: y_I = b_I - sum_{d=0..2} L_{I, I-3+d} y_{I-3+d} (ring slot of block J: J & 3)
+    for (int I = 0; I < ns; ++I) {
+        pr = Pr + I * ps + (long long)i * ld;
+        pi = CPLX ? Pi + I * ps + (long long)i * ld : nullptr;
+        const int d0 = I >= 3 ? 0 : 3 - I;  // blocks d < d0 lie left of the matrix
+        double ar = 0.0, ai = 0.0;
+        load(0, d0 <= 0, a0, b0);
+        load(m, d0 <= 1, a1, b1);
+        contract(a0, b0, rr + ((I + 1) & 3) * 64, ri + ((I + 1) & 3) * 64, ar, ai);
+        load(2 * m, d0 <= 2, a0, b0);
+        contract(a1, b1, rr + ((I + 2) & 3) * 64, ri + ((I + 2) & 3) * 64, ar, ai);
+        contract(a0, b0, rr + ((I + 3) & 3) * 64, ri + ((I + 3) & 3) * 64, ar, ai);
+        reduce4(ar, ai);
+        if (row && q == 0) {
+            const double vr = xr[I * m + i] - ar;
+            rr[(I & 3) * 64 + i] = vr;
+            xr[I * m + i] = vr;
+            if (CPLX) {
+                const double vi = xi[I * m + i] - ai;
+                ri[(I & 3) * 64 + i] = vi;
+                xi[I * m + i] = vi;
+            }
+        }
+        __syncthreads();
+    }
+    // backward: x_I = Ainv_I (y_I - sum_{d=0..2} U_{I, I+1+d} x_{I+1+d})
+    for (int I = ns - 1; I >= 0; --I) {
+        pr = Pr + I * ps + (long long)i * ld;
+        pi = CPLX ? Pi + I * ps + (long long)i * ld : nullptr;
+        const int nK = (ns - 1 - I) < 3 ? (ns - 1 - I) : 3;  // blocks right of the matrix are skipped
+        double ar = 0.0, ai = 0.0;
+        load(4 * m, nK > 0, a0, b0);
+        load(5 * m, nK > 1, a1, b1);
+        contract(a0, b0, rr + ((I + 1) & 3) * 64, ri + ((I + 1) & 3) * 64, ar, ai);
+        load(6 * m, nK > 2, a0, b0);
+        contract(a1, b1, rr + ((I + 2) & 3) * 64, ri + ((I + 2) & 3) * 64, ar, ai);
+        load(3 * m, true, a1, b1);  // Ainv_I row, used after the barrier
+        contract(a0, b0, rr + ((I + 3) & 3) * 64, ri + ((I + 3) & 3) * 64, ar, ai);
+        reduce4(ar, ai);
+        if (row && q == 0) {
+            tr[i] = xr[I * m + i] - ar;
+            if (CPLX) ti[i] = xi[I * m + i] - ai;
+        }
+        __syncthreads();
+        ar = 0.0, ai = 0.0;
+        contract(a1, b1, tr, ti, ar, ai);
+        reduce4(ar, ai);
+        if (row && q == 0) {
+            xr[I * m + i] = ar;
+            rr[(I & 3) * 64 + i] = ar;
+            if (CPLX) {
+                xi[I * m + i] = ai;
+                ri[(I & 3) * 64 + i] = ai;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Solve E x = b in place with the block factors. Real: NR right-hand sides
 // x[0..NR-1]; complex: one right-hand side (xr[0], xi[0]).
 // Thread (i, q): row i = tid / 4 of the current block, part q = tid % 4.
-template <bool CPLX, int NR>
-__device__ void block_solve(const double* __restrict__ Pr, const double* __restrict__ Pi, int ns, int m,
+template <bool CPLX, int NR, class T = double>
+__device__ void block_solve(const T* __restrict__ Pr, const T* __restrict__ Pi, int ns, int m,
                             double* const* xr, double* const* xi, double* sT /* LDS [2 * NR * MAX_BLOCK] */) {
     const int ld = 7 * m;
     const long long ps = (long long)m * ld;
@@ -454,8 +633,8 @@ __device__ void block_solve(const double* __restrict__ Pr, const double* __restr
     double* sTi = sT + NR * MAX_BLOCK;
     // forward: y_I = b_I - sum_{J=I-3}^{I-1} L_IJ y_J
     for (int I = 0; I < ns; ++I) {
-        const double* PIr = Pr + I * ps;
-        const double* PIi = Pi + I * ps;
+        const T* PIr = Pr + I * ps;
+        const T* PIi = Pi + I * ps;
         double ar[NR], ai[NR];
 #pragma unroll
         for (int v = 0; v < NR; ++v) ar[v] = 0.0, ai[v] = 0.0;
@@ -495,8 +674,8 @@ __device__ void block_solve(const double* __restrict__ Pr, const double* __restr
     }
     // backward: x_I = Ainv_I (y_I - sum_{K=I+1}^{I+3} U_IK x_K)
     for (int I = ns - 1; I >= 0; --I) {
-        const double* PIr = Pr + I * ps;
-        const double* PIi = Pi + I * ps;
+        const T* PIr = Pr + I * ps;
+        const T* PIi = Pi + I * ps;
         double ar[NR], ai[NR];
 #pragma unroll
         for (int v = 0; v < NR; ++v) ar[v] = 0.0, ai[v] = 0.0;

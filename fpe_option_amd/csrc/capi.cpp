@@ -16,6 +16,24 @@ namespace {
 
 thread_local std::string g_last_error;
 
+fpe::OptionSpec from_batch(const FpeBatchOption& b) {
+    fpe::OptionSpec o{};
+    o.kappa = b.kappa, o.theta = b.theta, o.eta = b.sigma, o.rho = b.rho, o.r = b.r, o.T = b.T;
+    o.S0 = b.S0, o.V0 = b.V0;
+    o.s_min = b.s_min >= 0.0 ? b.s_min : 0.0;
+    o.s_max = b.s_max > 0.0 ? b.s_max : 3.0 * b.S0;
+    o.v_max = b.v_max > 0.0 ? b.v_max : 1.0;
+    o.barrier = b.barrier, o.strike = b.strike, o.option_type = b.option_type;
+    o.n_s = b.n_s, o.n_v = b.n_v, o.disc_mode = b.disc_mode, o.num_insert = b.num_insert;
+    o.complex_mode = b.complex_mode, o.precision = b.precision, o.exact_constants = b.exact_constants;
+    o.ic_sigma_s = b.ic_sigma_s, o.ic_sigma_v = b.ic_sigma_v;
+    o.rtol = b.rtol > 0 ? b.rtol : 1e-4;
+    o.atol = b.atol > 0 ? b.atol : 1e-6;
+    o.knot_width_s = b.knot_width_s, o.knot_width_v = b.knot_width_v;
+    o.knot_center_v = b.knot_center_v;
+    return o;
+}
+
 double* heap_copy(const std::vector<double>& v) {
     if (v.empty()) return nullptr;
     double* p = static_cast<double*>(std::malloc(v.size() * sizeof(double)));
@@ -331,22 +349,7 @@ int32_t fpe_batch_price(const FpeBatchOption* opts, int32_t n, double* prices, i
         return 1;
     }
     std::vector<fpe::OptionSpec> v(n);
-    for (int i = 0; i < n; ++i) {
-        const FpeBatchOption& b = opts[i];
-        fpe::OptionSpec& o = v[i];
-        o.kappa = b.kappa, o.theta = b.theta, o.eta = b.sigma, o.rho = b.rho, o.r = b.r, o.T = b.T;
-        o.S0 = b.S0, o.V0 = b.V0;
-        o.s_min = b.s_min >= 0.0 ? b.s_min : 0.0;
-        o.s_max = b.s_max > 0.0 ? b.s_max : 3.0 * b.S0;
-        o.v_max = b.v_max > 0.0 ? b.v_max : 1.0;
-        o.barrier = b.barrier, o.strike = b.strike, o.option_type = b.option_type;
-        o.n_s = b.n_s, o.n_v = b.n_v, o.disc_mode = b.disc_mode, o.num_insert = b.num_insert;
-        o.complex_mode = b.complex_mode, o.precision = b.precision, o.exact_constants = b.exact_constants;
-        o.ic_sigma_s = b.ic_sigma_s, o.ic_sigma_v = b.ic_sigma_v;
-        o.rtol = b.rtol > 0 ? b.rtol : 1e-4;
-        o.atol = b.atol > 0 ? b.atol : 1e-6;
-        o.knot_width_s = b.knot_width_s, o.knot_width_v = b.knot_width_v;
-    }
+    for (int i = 0; i < n; ++i) v[i] = from_batch(opts[i]);
     std::vector<double> p;
     std::vector<int> s;
     std::vector<fpe::SolveResult> res;
@@ -360,6 +363,7 @@ int32_t fpe_batch_price(const FpeBatchOption* opts, int32_t n, double* prices, i
             stats->n_accepted += r.n_acc, stats->n_rejected += r.n_rej, stats->n_lu += r.n_lu;
             stats->n_newton += r.n_newt, stats->n_err_solves += r.n_err, stats->n_inner_iters += r.n_inner;
             stats->n_admm_iters += r.n_admm;
+            stats->n_refine += r.n_refine;
         }
         for (auto& r : res) {
             const double nb = r.nsb, m = r.nvb, N = nb * m;
@@ -427,8 +431,113 @@ void fpe_compute_free_csr(struct FpeCsrResult* r) {
     r->data = nullptr, r->indices = nullptr, r->indptr = nullptr;
 }
 
+int32_t fpe_batch_terminal_states(const FpeBatchOption* opts, int32_t n, double* qT, int64_t ldq, int32_t* n_dof,
+                                  int32_t* status, FpeBatchStats* stats) {
+    g_last_error.clear();
+    if (n < 0 || (n > 0 && (!opts || !qT || !n_dof || !status)) || ldq <= 0) {
+        g_last_error = "invalid arguments";
+        return 1;
+    }
+    std::vector<fpe::SolveSpec> solves;
+    std::vector<fpe::Domain> doms;
+    std::vector<int> which(n, -1);
+    for (int i = 0; i < n; ++i) {
+        n_dof[i] = 0;
+        status[i] = 1;
+        fpe::OptionSpec o = from_batch(opts[i]);
+        std::string why;
+        if ((o.option_type != 8 && o.option_type != 9) || !fpe::option_is_valid(o, &why)) continue;
+        fpe::SolveSpec sp = fpe::solve_for(o, o.option_type);
+        fpe::Domain d = fpe::build_domain(sp);
+        if ((int64_t)d.n_sb * d.n_vb > ldq) continue;
+        which[i] = int(solves.size());
+        solves.push_back(sp);
+        doms.push_back(std::move(d));
+    }
+    std::vector<fpe::SolveResult> res;
+    fpe::BatchTimes bt;
+    std::vector<double> dummy;
+    fpe::RunOptions ro;
+    ro.want_qT = true;
+    if (!solves.empty() && !fpe::run_batch(solves, doms, {}, 0, ro, dummy, res, bt, g_last_error)) return 2;
+    for (int i = 0; i < n; ++i) {
+        if (which[i] < 0) continue;
+        const fpe::SolveResult& r = res[which[i]];
+        status[i] = r.status;
+        n_dof[i] = int32_t(r.qT.size());
+        std::copy(r.qT.begin(), r.qT.end(), qT + (int64_t)i * ldq);
+    }
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->n_solves = int64_t(res.size());
+        for (auto& r : res) {
+            stats->n_accepted += r.n_acc, stats->n_rejected += r.n_rej, stats->n_lu += r.n_lu;
+            stats->n_newton += r.n_newt, stats->n_err_solves += r.n_err, stats->n_inner_iters += r.n_inner;
+            stats->n_admm_iters += r.n_admm;
+        }
+        stats->ms_setup = bt.setup, stats->ms_assemble = bt.assemble, stats->ms_ic = bt.ic;
+        stats->ms_radau = bt.radau, stats->ms_payoff = bt.payoff, stats->ms_total = bt.total;
+    }
+    return 0;
+}
+
+int32_t fpe_european_pricing_operator(const FpeBatchOption* opt, int32_t cap_qs, int32_t cap_dof, double* s_phys,
+                                      double* marg, int32_t* n_qs, int32_t* n_dof) {
+    g_last_error.clear();
+    if (!opt || !s_phys || !marg || !n_qs || !n_dof) {
+        g_last_error = "invalid arguments";
+        return 1;
+    }
+    fpe::OptionSpec o = from_batch(*opt);
+    std::string why;
+    if ((o.option_type != 8 && o.option_type != 9) || !fpe::option_is_valid(o, &why)) {
+        g_last_error = "a European option (type 8 or 9) with valid parameters is required: " + why;
+        return 1;
+    }
+    fpe::SolveSpec sp = fpe::solve_for(o, o.option_type);
+    fpe::Domain d = fpe::build_domain(sp);
+    const int nqs = int(d.s_phys.size()), nqv = int(d.v_w.size()), ns = d.n_sb, nv = d.n_vb;
+    *n_qs = nqs, *n_dof = ns * nv;
+    if (nqs > cap_qs || ns * nv > cap_dof) {
+        g_last_error = "capacity too small";
+        return 1;
+    }
+    std::vector<fpe::SolveResult> res;
+    fpe::BatchTimes bt;
+    std::vector<double> dummy;
+    fpe::RunOptions ro;
+    ro.want_basis = true;
+    ro.stop_after_ic = true;
+    if (!fpe::run_batch({sp}, {d}, {}, 0, ro, dummy, res, bt, g_last_error)) return 2;
+    const std::vector<double>& Ts = res[0].bas[0];
+    const std::vector<double>& Tv = res[0].bas[1];
+    const std::vector<int>& cs = res[0].bas_c0[0];
+    const std::vector<int>& cv = res[0].bas_c0[1];
+    std::vector<double> om(nv, 0.0);  // omega_l = sum_j Bv[j][l] w_v,j
+    for (int j = 0; j < nqv; ++j)
+        for (int t = 0; t < 4; ++t) {
+            const int l = cv[j] + t;
+            if (l >= 0 && l < nv) om[l] += Tv[8LL * j + t] * d.v_w[j];
+        }
+    std::fill(marg, marg + (int64_t)nqs * ns * nv, 0.0);
+    for (int i = 0; i < nqs; ++i) {
+        s_phys[i] = d.s_phys[i];
+        for (int t = 0; t < 4; ++t) {
+            const int k = cs[i] + t;
+            if (k < 0 || k >= ns) continue;
+            const double a = d.s_w[i] * Ts[8LL * i + t];
+            for (int l = 0; l < nv; ++l) marg[(int64_t)i * ns * nv + (int64_t)k * nv + l] += a * om[l];
+        }
+    }
+    return 0;
+}
+
 const char* fpe_last_error(void) { return g_last_error.c_str(); }
 
 int32_t fpe_device_count(void) { return fpe::device_count(); }
 
 }  // extern "C"
+
+namespace fpe {
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+}  // namespace fpe

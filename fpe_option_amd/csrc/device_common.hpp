@@ -40,7 +40,7 @@ struct Inst {
 // Per-instance result counters.
 struct InstOut {
     int status, n_acc, n_rej, n_lu, n_newt, n_err, n_inner, n_admm;
-    int n_hist, pad;
+    int n_hist, n_refine;  // n_refine: FP64 refinement rounds of the mixed-precision stage solves
     double t_end, h_last;
     // wall-clock ticks (100 MHz) spent per phase by the solver workgroup
     unsigned long long tk_asm, tk_lu, tk_solve, tk_mv, tk_other, tk_total;

@@ -206,6 +206,7 @@ SolveSpec solve_for(const OptionSpec& o, int t) {
     s.ic_sigma_s = o.ic_sigma_s, s.ic_sigma_v = o.ic_sigma_v, s.rtol = o.rtol, s.atol = o.atol;
     s.complex_mode = o.complex_mode, s.precision = o.precision, s.exact_constants = o.exact_constants;
     s.knot_width_s = o.knot_width_s, s.knot_width_v = o.knot_width_v;
+    s.knot_center_v = o.knot_center_v;
     return s;
 }
 
@@ -223,7 +224,8 @@ Domain build_domain(const SolveSpec& s) {
         const double ws = s.knot_width_s > 0 ? s.knot_width_s : 0.1;
         const double wv = s.knot_width_v > 0 ? s.knot_width_v : 0.1;
         std::vector<double> bs = stretched_breakpoints(s.n_s, cs, ws);
-        std::vector<double> bv = stretched_breakpoints(s.n_v, s.V0 / s.v_hi, wv);
+        const double cv = s.knot_center_v > 0 ? s.knot_center_v : s.V0;
+        std::vector<double> bv = stretched_breakpoints(s.n_v, cv / s.v_hi, wv);
         for (double& x : bv) x *= s.v_hi;
         bv.back() = s.v_hi;
         d.s_knots.assign(3, 0.0);

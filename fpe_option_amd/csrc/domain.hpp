@@ -22,6 +22,7 @@ struct OptionSpec {
     int complex_mode, precision, exact_constants;
     double ic_sigma_s, ic_sigma_v, rtol, atol;
     double knot_width_s, knot_width_v;
+    double knot_center_v;  // span mode: v breakpoints cluster here (<= 0: at V0)
 };
 
 // One Fokker-Planck solve on a (possibly barrier-truncated) domain.
@@ -33,6 +34,7 @@ struct SolveSpec {
     double ic_sigma_s, ic_sigma_v, rtol, atol;
     int complex_mode, precision, exact_constants;
     double knot_width_s, knot_width_v;
+    double knot_center_v;
 };
 
 // Knots and quadrature of one solve.

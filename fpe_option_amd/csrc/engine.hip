@@ -173,7 +173,7 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
         max_N = std::max(max_N, N);
         max_nm = std::max(max_nm, N);
         max_m = std::max(max_m, I.nvb);
-        any_direct |= (sp.complex_mode == 1);
+        any_direct |= (sp.complex_mode == 1 || sp.precision == 1);
     }
     max_panel = (max_panel + 31) & ~31LL;  // panels start 256-byte aligned (16-byte stores in assembly)
     SlotLayout L{};
@@ -269,9 +269,9 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
         SolveResult& R = results[s];
         const InstOut& o = ho[s];
         R.status = o.status, R.n_acc = o.n_acc, R.n_rej = o.n_rej, R.n_lu = o.n_lu, R.n_newt = o.n_newt;
-        R.n_err = o.n_err, R.n_inner = o.n_inner, R.n_admm = o.n_admm;
+        R.n_err = o.n_err, R.n_inner = o.n_inner, R.n_admm = o.n_admm, R.n_refine = o.n_refine;
         R.nsb = hi[s].nsb, R.nvb = hi[s].nvb;
-        R.direct = hi[s].cmode == 1;
+        R.direct = hi[s].cmode == 1 || hi[s].prec == 1;
         const unsigned long long tks[6] = {o.tk_asm, o.tk_lu, o.tk_solve, o.tk_mv, o.tk_other, o.tk_total};
         for (int k = 0; k < 6; ++k) R.tk[k] = double(tks[k]) * 1e-8;
         for (int k = 0; k < 4; ++k) R.tk[6 + k] = double(o.tk_lu_part[k]) * 1e-8;

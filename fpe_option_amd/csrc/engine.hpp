@@ -11,7 +11,7 @@
 namespace fpe {
 
 struct SolveResult {
-    int status = 0, n_acc = 0, n_rej = 0, n_lu = 0, n_newt = 0, n_err = 0, n_inner = 0, n_admm = 0;
+    int status = 0, n_acc = 0, n_rej = 0, n_lu = 0, n_newt = 0, n_err = 0, n_inner = 0, n_admm = 0, n_refine = 0;
     int nsb = 0, nvb = 0;
     bool direct = false;
     double tk[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // seconds: assemble, lu, solve, mv, other, total, then factorisation parts: pivot inverse, staging, L multiply, Schur (solver workgroup)
@@ -50,5 +50,8 @@ bool price_options(const std::vector<OptionSpec>& opts, std::vector<double>& pri
                    std::vector<SolveResult>* results, BatchTimes* times, std::string& err);
 
 int device_count();
+
+// Sets the message fpe_last_error() returns in this thread.
+void set_last_error(const std::string& msg);
 
 }  // namespace fpe

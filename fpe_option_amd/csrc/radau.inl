@@ -46,6 +46,7 @@ constexpr int GM_K = 10;  // GMRES Krylov dimension (radau.mojo:81-82)
 enum {
     S_Y = 0, S_W, S_SCAL, S_Z1, S_Z2, S_Z3, S_F1, S_F2, S_F3, S_KZ1, S_KZ2, S_KZ3, S_MF1, S_MF2, S_MF3,
     S_RR, S_RRE, S_RIM, S_DF1, S_DF2, S_DF3, S_C0, S_C1, S_C2, S_C3, S_CERR, S_MC, S_ERR, S_TMP, S_WRE, S_WIM,
+    S_RB, S_RBI, S_RT, S_RTI,  // iterative refinement (mixed precision): right-hand side, residual/correction
     S_VRE0, S_VIM0 = S_VRE0 + GM_K + 1, S_NVEC = S_VIM0 + GM_K + 1
 };
 
@@ -141,6 +142,42 @@ __device__ void gmres_mv(const double* __restrict__ Ms, const double* __restrict
     }
 }
 
+// Residual of a stage system in FP64 from the stencils: r = b - E x with
+// E = (a + i beta) M + h K (beta = 0 and imaginary parts unused when !cplx).
+__device__ void stage_residual(const double* __restrict__ Ms, const double* __restrict__ Ks, int ns, int nv, double a,
+                               double beta, double h, const double* xr, const double* xi, const double* br,
+                               const double* bi, double* rr, double* ri, bool cplx) {
+    const int N = ns * nv;
+    for (int r = threadIdx.x; r < N; r += NT) {
+        const int I = r / nv, i = r - I * nv;
+        double er = 0, ei = 0, mr = 0, mi = 0;
+        for (int dJ = -3; dJ <= 3; ++dJ) {
+            const int J = I + dJ;
+            if (J < 0 || J >= ns) continue;
+            for (int dj = -3; dj <= 3; ++dj) {
+                const int jj = i + dj;
+                if (jj < 0 || jj >= nv) continue;
+                const long long e = (long long)((dJ + 3) * 7 + dj + 3) * N + r;
+                const int c = J * nv + jj;
+                const double mv = Ms[e];
+                const double ed = a * mv + h * Ks[e];
+                er += ed * xr[c];
+                if (cplx) {
+                    ei += ed * xi[c];
+                    mr += mv * xr[c];
+                    mi += mv * xi[c];
+                }
+            }
+        }
+        if (cplx) {
+            rr[r] = br[r] - (er - beta * mi);
+            ri[r] = bi[r] - (ei + beta * mr);
+        } else {
+            rr[r] = br[r] - er;
+        }
+    }
+}
+
 __global__ void __launch_bounds__(NT) k_radau(const Inst* __restrict__ inst, int n_inst, int* __restrict__ queue,
                                               const double* __restrict__ ST, double* __restrict__ VC,
                                               double* __restrict__ WS, int* __restrict__ WI, SlotLayout L,
@@ -168,7 +205,12 @@ __global__ void __launch_bounds__(NT) k_radau(const Inst* __restrict__ inst, int
         double* Ci = ws + L.o_ci;
         double* const vb = ws + L.o_v;
         auto vv = [=](int k) { return vb + (long long)k * N; };
-        const bool direct = in.cmode == 1;
+        const bool mixed = in.prec == 1;
+        // mixed precision always solves the complex stage system directly (FP32 factors, FP64 refinement)
+        const bool direct = in.cmode == 1 || mixed;
+        float* Pf = reinterpret_cast<float*>(P);
+        float* Crf = reinterpret_cast<float*>(Cr);
+        float* Cif = reinterpret_cast<float*>(Ci);
         const double U1 = in.exact ? 3.637834252744496 : 3.6378165692476072;
         const double ALPH = in.exact ? 2.681082873627752 : 2.6753213032678365;
         const double BETA = in.exact ? 3.050430199247411 : 3.0493570545593676;
@@ -210,6 +252,66 @@ __global__ void __launch_bounds__(NT) k_radau(const Inst* __restrict__ inst, int
         const double sa = fmax(atol_w, 1e-300), sr = fmax(rtol_w, 1e-300);
         long long guard = 0;
 
+        // Stage solves. FP64: one block solve with the FP64 factors. Mixed: the FP32
+        // factors give x0; FP64 residuals against the factored matrix E(h_lu) (from
+        // the stencils) drive corrections until |r|_inf <= 1e-12 |b|_inf (at most
+        // 10 rounds), so the result is the FP64 path's solve up to that residual.
+        double* ring = S.lu.B[0];  // the factorisation's staging LDS is idle while solving
+        constexpr int REFINE_MAX = 10;
+        constexpr double REFINE_TOL = 1e-12;
+        int n_ref = 0;
+        auto solve_real = [&](double* x) {
+            if (!mixed) {
+                block_solve1<false, double>(P, nullptr, ns, m, x, nullptr, ring);
+                return;
+            }
+            double* b = vv(S_RB);
+            double* tv = vv(S_RT);
+            double bn = 0.0;
+            for (int r = tid; r < N; r += NT) b[r] = x[r], bn = fmax(bn, fabs(x[r]));
+            bn = block_max(bn, S.red);
+            block_solve1<false, float>(Pf, nullptr, ns, m, x, nullptr, ring);
+            for (int it = 0; it < REFINE_MAX; ++it) {
+                stage_residual(Ms, Ks, ns, nv, U1, 0.0, h_lu, x, nullptr, b, nullptr, tv, nullptr, false);
+                double rn = 0.0;
+                for (int r = tid; r < N; r += NT) rn = fmax(rn, fabs(tv[r]));
+                rn = block_max(rn, S.red);
+                if (!(rn > REFINE_TOL * bn)) break;
+                ++n_ref;
+                block_solve1<false, float>(Pf, nullptr, ns, m, tv, nullptr, ring);
+                for (int r = tid; r < N; r += NT) x[r] += tv[r];
+                __syncthreads();
+            }
+        };
+        auto solve_cplx = [&](double* xr, double* xi) {
+            if (!mixed) {
+                block_solve1<true, double>(Cr, Ci, ns, m, xr, xi, ring);
+                return;
+            }
+            double* br = vv(S_RB);
+            double* bi = vv(S_RBI);
+            double* tr = vv(S_RT);
+            double* ti = vv(S_RTI);
+            double bn = 0.0;
+            for (int r = tid; r < N; r += NT) {
+                br[r] = xr[r], bi[r] = xi[r];
+                bn = fmax(bn, fmax(fabs(xr[r]), fabs(xi[r])));
+            }
+            bn = block_max(bn, S.red);
+            block_solve1<true, float>(Crf, Cif, ns, m, xr, xi, ring);
+            for (int it = 0; it < REFINE_MAX; ++it) {
+                stage_residual(Ms, Ks, ns, nv, ALPH, BETA, h_lu, xr, xi, br, bi, tr, ti, true);
+                double rn = 0.0;
+                for (int r = tid; r < N; r += NT) rn = fmax(rn, fmax(fabs(tr[r]), fabs(ti[r])));
+                rn = block_max(rn, S.red);
+                if (!(rn > REFINE_TOL * bn)) break;
+                ++n_ref;
+                block_solve1<true, float>(Crf, Cif, ns, m, tr, ti, ring);
+                for (int r = tid; r < N; r += NT) xr[r] += tr[r], xi[r] += ti[r];
+                __syncthreads();
+            }
+        };
+
         while (posneg * (t1 - t) > uround * fmax(fabs(t), fabs(t1))) {
             if (n_steps > 100000 || ++guard > 2000000) { status = 2; break; }
             if (posneg * (t + 1.01 * h - t1) > 0.0) h = t1 - t;
@@ -221,17 +323,31 @@ __global__ void __launch_bounds__(NT) k_radau(const Inst* __restrict__ inst, int
             }
             if (need_lu) {
                 tk_mark = wall_clock64();
-                assemble_E<false>(Ms, Ks, ns, m, U1, 0.0, h, P, nullptr);
-                __syncthreads();
-                lap(tk_asm);
-                int bad = block_lu<false>(P, nullptr, ns, m, S.lu);
-                lap(tk_lu);
-                if (direct) {
-                    assemble_E<true>(Ms, Ks, ns, m, ALPH, BETA, h, Cr, Ci);
+                int bad = 0;
+                if (mixed) {
+                    assemble_E<false, float>(Ms, Ks, ns, m, U1, 0.0, h, Pf, (float*)nullptr);
                     __syncthreads();
                     lap(tk_asm);
-                    bad |= block_lu<true>(Cr, Ci, ns, m, S.lu);
+                    bad = block_lu<false, float>(Pf, (float*)nullptr, ns, m, S.lu);
                     lap(tk_lu);
+                    assemble_E<true, float>(Ms, Ks, ns, m, ALPH, BETA, h, Crf, Cif);
+                    __syncthreads();
+                    lap(tk_asm);
+                    bad |= block_lu<true, float>(Crf, Cif, ns, m, S.lu);
+                    lap(tk_lu);
+                } else {
+                    assemble_E<false, double>(Ms, Ks, ns, m, U1, 0.0, h, P, (double*)nullptr);
+                    __syncthreads();
+                    lap(tk_asm);
+                    bad = block_lu<false, double>(P, (double*)nullptr, ns, m, S.lu);
+                    lap(tk_lu);
+                    if (direct) {
+                        assemble_E<true, double>(Ms, Ks, ns, m, ALPH, BETA, h, Cr, Ci);
+                        __syncthreads();
+                        lap(tk_asm);
+                        bad |= block_lu<true, double>(Cr, Ci, ns, m, S.lu);
+                        lap(tk_lu);
+                    }
                 }
                 (void)bad;
                 h_lu = h;
@@ -285,16 +401,13 @@ __global__ void __launch_bounds__(NT) k_radau(const Inst* __restrict__ inst, int
                 __syncthreads();
                 lap(tk_mv);
                 if (direct) {
-                    double* xr1[1] = {vv(S_DF1)};
-                    block_solve<false, 1>(P, nullptr, ns, m, xr1, nullptr, S.sT);
-                    double* xr2[1] = {vv(S_DF2)};
-                    double* xi2[1] = {vv(S_DF3)};
-                    block_solve<true, 1>(Cr, Ci, ns, m, xr2, xi2, S.sT);
+                    solve_real(vv(S_DF1));
+                    solve_cplx(vv(S_DF2), vv(S_DF3));
                     lap(tk_solve);
                 } else {
                     // dF1 and the preconditioned GMRES right-hand side in one pass
                     double* xr3[3] = {vv(S_DF1), vv(S_VRE0), vv(S_VIM0)};
-                    block_solve<false, 3>(P, nullptr, ns, m, xr3, nullptr, S.sT);
+                    block_solve<false, 3, double>(P, nullptr, ns, m, xr3, nullptr, S.sT);
                     // ---- GMRES(10), radau.mojo:916-1146
                     double bsq = 0.0;
                     for (int r = tid; r < N; r += NT) {
@@ -319,7 +432,7 @@ __global__ void __launch_bounds__(NT) k_radau(const Inst* __restrict__ inst, int
                                      vv(S_WIM));
                             __syncthreads();
                             double* xw[2] = {vv(S_WRE), vv(S_WIM)};
-                            block_solve<false, 2>(P, nullptr, ns, m, xw, nullptr, S.sT);
+                            block_solve<false, 2, double>(P, nullptr, ns, m, xw, nullptr, S.sT);
                             for (int k = 0; k <= j; ++k) {
                                 double hv = 0.0;
                                 for (int r = tid; r < N; r += NT)
@@ -460,8 +573,7 @@ __global__ void __launch_bounds__(NT) k_radau(const Inst* __restrict__ inst, int
             __syncthreads();
             {
                 tk_mark = wall_clock64();
-                double* xe[1] = {vv(S_ERR)};
-                block_solve<false, 1>(P, nullptr, ns, m, xe, nullptr, S.sT);
+                solve_real(vv(S_ERR));
                 lap(tk_solve);
             }
             ++n_err;
@@ -480,8 +592,7 @@ __global__ void __launch_bounds__(NT) k_radau(const Inst* __restrict__ inst, int
                 __syncthreads();
                 for (int r = tid; r < N; r += NT) vv(S_ERR)[r] = vv(S_MC)[r] - h * vv(S_W)[r];
                 __syncthreads();
-                double* xe[1] = {vv(S_ERR)};
-                block_solve<false, 1>(P, nullptr, ns, m, xe, nullptr, S.sT);
+                solve_real(vv(S_ERR));
                 ++n_err;
                 en = 0.0;
                 for (int r = tid; r < N; r += NT) {
@@ -535,6 +646,7 @@ __global__ void __launch_bounds__(NT) k_radau(const Inst* __restrict__ inst, int
             o.status = status;
             o.n_acc = n_acc, o.n_rej = n_rej, o.n_lu = n_lu, o.n_newt = n_newt, o.n_err = n_err;
             o.n_inner = n_inner, o.n_hist = nh, o.t_end = t, o.h_last = h;
+            o.n_refine = n_ref;
             o.tk_asm = tk_asm, o.tk_lu = tk_lu, o.tk_solve = tk_solve, o.tk_mv = tk_mv;
             o.tk_total = wall_clock64() - tk_start;
             o.tk_other = o.tk_total - tk_asm - tk_lu - tk_solve - tk_mv;

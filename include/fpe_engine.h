@@ -159,6 +159,7 @@ typedef struct FpeBatchOption {
     double ic_sigma_s, ic_sigma_v; /* <= 0 -> reference */
     double rtol, atol;             /* <= 0 -> 1e-4 / 1e-6 */
     double knot_width_s, knot_width_v; /* span mode, <= 0 -> 0.1 */
+    double knot_center_v;              /* span mode: v breakpoints cluster here; <= 0 -> V0 */
 } FpeBatchOption;
 
 /** Per-batch counters (summed over the solves of the batch). */
@@ -176,6 +177,8 @@ typedef struct FpeBatchStats {
     double wg_s_assemble, wg_s_lu, wg_s_solve, wg_s_spmv, wg_s_other, wg_s_total;
     /* Factorisation seconds split: pivot-block inverse, operand staging, L multiply, Schur update. */
     double wg_s_lu_inverse, wg_s_lu_stage, wg_s_lu_lmul, wg_s_lu_schur;
+    /* FP64 refinement rounds of the mixed-precision stage solves (FPE_PREC_MIXED). */
+    int64_t n_refine;
 } FpeBatchStats;
 
 /** CSR matrix (row-major, sorted columns). */
@@ -201,6 +204,72 @@ void fpe_batch_option_default(FpeBatchOption* o);
  *  Returns 0 on success, nonzero if the batch could not run (see fpe_last_error). */
 int32_t fpe_batch_price(const FpeBatchOption* opts, int32_t n, double* prices, int32_t* status,
                         FpeBatchStats* stats);
+
+/** Terminal densities q(T) (recombined B-spline coefficients, dof r = I n_vb + i)
+ *  of n European options (types 8, 9; strike and type do not enter q(T)) on
+ *  their full domains. Row i of qT (leading dimension ldq) receives n_dof[i]
+ *  values; status[i] as in fpe_batch_price (FPE_INVALID for other types or
+ *  when n_dof exceeds ldq). Returns 0 when the batch ran. */
+int32_t fpe_batch_terminal_states(const FpeBatchOption* opts, int32_t n, double* qT, int64_t ldq, int32_t* n_dof,
+                                  int32_t* status, FpeBatchStats* stats);
+
+/** Pricing operator of one European domain: the s quadrature points s_phys
+ *  (n_qs) and the marginal operator marg (n_qs x n_dof, row-major) with
+ *  price = exp(-r T) sum_i payoff(s_phys[i]) (marg q(T))_i, i.e.
+ *  marg[i][I n_vb + l] = w_s,i Bs[i][I] sum_j Bv[j][l] w_v,j. Capacities:
+ *  cap_qs points, cap_dof dofs. Returns 0 on success. */
+int32_t fpe_european_pricing_operator(const FpeBatchOption* opt, int32_t cap_qs, int32_t cap_dof, double* s_phys,
+                                      double* marg, int32_t* n_qs, int32_t* n_dof);
+
+/* ---- surrogate kernels (no counterpart in the reference). Device pointers
+ * (row-major), work queued on `stream` (a hipStream_t; NULL = the null stream).
+ * Return 0, or nonzero on a launch error (see fpe_last_error). */
+
+/** Z = X W^T + b (M x N) and A = silu(Z) (act = 1) or A = Z (act = 0); X is M x K
+ *  FP32 (rounded to BF16 as an MFMA operand), W is N x K BF16, accumulation FP32.
+ *  Z or A may be NULL. */
+int32_t fpe_mlp_forward(const float* X, int32_t M, int32_t K, const uint16_t* Wb, const float* b, int32_t N,
+                        int32_t act, float* Z, float* A, void* stream);
+
+/** Backward of fpe_mlp_forward given dZ (M x N): dW = dZ^T X (N x K), db = column
+ *  sums of dZ, dX = dZ W (M x K) times silu'(Zp) when Zp is not NULL. dX may be
+ *  NULL. BF16 operands, FP32 accumulation. */
+int32_t fpe_mlp_backward(const float* dZ, const float* X, const uint16_t* Wb, int32_t M, int32_t N, int32_t K,
+                         float* dW, float* db, float* dX, const float* Zp, void* stream);
+
+/** Adam step (bias-corrected, step t >= 1) on FP32 master weights W (n), then
+ *  Wb = BF16(W) when Wb is not NULL. */
+int32_t fpe_mlp_adam(float* W, uint16_t* Wb, const float* G, float* m, float* v, int64_t n, float lr, float beta1,
+                     float beta2, float eps, int32_t t, void* stream);
+
+/** Weighted mean-square loss over M x N: dY = 2 wcol[c] (Y - T) / (M N) (wcol
+ *  may be NULL = 1); partial[0..1023] receives per-block sums of
+ *  wcol[c] (Y - T)^2 / (M N), all 1024 entries written. */
+int32_t fpe_mlp_mse(const float* Y, const float* T, const float* wcol, int32_t M, int32_t N, float* dY,
+                    float* partial, void* stream);
+
+/** dst[i][:] = src[idx[i]][:] for n rows of `width` floats (mini-batch gather). */
+int32_t fpe_mlp_gather_rows(const float* src, const int32_t* idx, int32_t n, int32_t width, float* dst,
+                            void* stream);
+
+/** Device memory for callers without a tensor library: allocation (NULL on
+ *  failure), release, synchronous copies, device-wide synchronisation. */
+void* fpe_device_alloc(int64_t bytes);
+void fpe_device_free(void* p);
+int32_t fpe_device_upload(void* dst, const void* src, int64_t bytes);
+int32_t fpe_device_download(void* dst, const void* src, int64_t bytes);
+int32_t fpe_device_synchronize(void);
+
+/** BF16 rounding of n FP32 values. */
+int32_t fpe_mlp_to_bf16(const float* W, uint16_t* Wb, int64_t n, void* stream);
+
+/** Surrogate prices from normalised POD coefficients C (n x ncoef, ncoef <= 64,
+ *  a multiple of 4): c = C * cscale + cshift; marginal w = w0 + P c with
+ *  Pt = P^T (ncoef x nq); price = exp(-r T) sum_i payoff(type, K, sq[i]) w_i
+ *  for European types 8 (call) and 9 (put). */
+int32_t fpe_surrogate_price(const float* C, int32_t n, int32_t ncoef, const float* cscale, const float* cshift,
+                            const float* Pt, const float* w0, const float* sq, int32_t nq, const double* strike,
+                            const int32_t* type, const double* r, const double* T, double* prices, void* stream);
 
 /** Human-readable description of the last failure in this thread (empty if none). */
 const char* fpe_last_error(void);
