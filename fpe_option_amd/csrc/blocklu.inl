@@ -19,8 +19,12 @@
 
 namespace fpe {
 
+// The Radau workgroup runs 8 waves (2 per SIMD), so one wave's memory or
+// barrier wait overlaps the other's work on the same SIMD.
+constexpr int RNT = 512;
+constexpr int RNW = RNT / 64;
 constexpr int LS = MAX_BLOCK + 1;  // LDS row stride of an m x m block
-constexpr int NRG = MAX_BLOCK / NWAVE;  // register columns per lane in the Gauss-Jordan inverse
+constexpr int NRG = MAX_BLOCK / RNW;  // register columns per lane in the Gauss-Jordan inverse
 
 struct LuLds {
     // four m x m staging blocks: the pivot block and its inverse (0, 1 = re, im),
@@ -150,7 +154,7 @@ __device__ void gj_inverse(LuLds& L, int m) {
     T xr[NRG], xi[NRG];
 #pragma unroll
     for (int r = 0; r < NRG; ++r) {
-        const int j = NWAVE * r + w;
+        const int j = RNW * r + w;
         xr[r] = (lrow && j < m) ? B0[lane * LS + j] : T(0);
         xi[r] = (CPLX && lrow && j < m) ? B1[lane * LS + j] : T(0);
     }
@@ -215,8 +219,8 @@ __device__ void gj_inverse(LuLds& L, int m) {
             }
         };
         const int k1 = k + 1;
-        if (k1 < m && w == (k1 & (NWAVE - 1))) {
-            const int r1 = k1 / NWAVE;
+        if (k1 < m && w == (k1 & (RNW - 1))) {
+            const int r1 = k1 / RNW;
             T cr = T(0), ci = T(0);
 #pragma unroll
             for (int r = 0; r < NRG; ++r)
@@ -226,13 +230,13 @@ __device__ void gj_inverse(LuLds& L, int m) {
         }
 #pragma unroll
         for (int r = 0; r < NRG; ++r) {
-            const int j = NWAVE * r + w;
+            const int j = RNW * r + w;
             if (j < m) upd(xr[r], xi[r], j);
         }
     }
 #pragma unroll
     for (int r = 0; r < NRG; ++r) {
-        const int j = NWAVE * r + w;
+        const int j = RNW * r + w;
         if (lrow && j < m) {
             B2[lane * LS + j] = xr[r];
             if (CPLX) B3[lane * LS + j] = xi[r];
@@ -241,7 +245,7 @@ __device__ void gj_inverse(LuLds& L, int m) {
     __syncthreads();
     if (threadIdx.x < m) L.qinv[L.piv[threadIdx.x]] = threadIdx.x;
     __syncthreads();
-    for (int e = threadIdx.x; e < m * m; e += NT) {
+    for (int e = threadIdx.x; e < m * m; e += RNT) {
         const int a = e / m, c = e - a * m;
         const int src = L.piv[a] * LS + L.qinv[c];
         B0[a * LS + c] = B2[src];
@@ -270,7 +274,7 @@ __device__ void lower_multiply(T* __restrict__ Pr, T* __restrict__ Pi, long long
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, l4 = lane >> 4;
     const int M = nU * m, tm = (M + 15) >> 4, tk = (m + 3) >> 2, tn = (m + 15) >> 4;
-    for (int t = wave; t < tm; t += NWAVE) {
+    for (int t = wave; t < tm; t += RNW) {
         const int r0 = t << 4, ra = r0 + l16;
         const bool va = ra < M;
         int dJa;
@@ -329,7 +333,7 @@ __device__ void schur_pass(T* __restrict__ Pr, T* __restrict__ Pi, long long ps,
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, l4 = lane >> 4;
     const int M = nU * m, tm = (M + 15) >> 4, tk = (m + 3) >> 2, tn = (m + 15) >> 4;
-    for (int t = wave; t < tm; t += NWAVE) {
+    for (int t = wave; t < tm; t += RNW) {
         const int r0 = t << 4, ra = r0 + l16;
         const bool va = ra < M;
         int dJa;
@@ -402,16 +406,16 @@ __device__ void assemble_E(const double* __restrict__ Ms, const double* __restri
     constexpr int V = int(16 / sizeof(T));  // elements per 16-byte store
     const long long nvec = tot / V;
     const double2 z = {0.0, 0.0};
-    for (long long e = threadIdx.x; e < nvec; e += NT) {
+    for (long long e = threadIdx.x; e < nvec; e += RNT) {
         reinterpret_cast<double2*>(Pr)[e] = z;
         if (CPLX) reinterpret_cast<double2*>(Pi)[e] = z;
     }
-    for (long long e = nvec * V + threadIdx.x; e < tot; e += NT) {
+    for (long long e = nvec * V + threadIdx.x; e < tot; e += RNT) {
         Pr[e] = T(0);
         if (CPLX) Pi[e] = T(0);
     }
     __syncthreads();
-    for (int row = threadIdx.x; row < N; row += NT) {
+    for (int row = threadIdx.x; row < N; row += RNT) {
         const int I = row / m, i = row - I * m;
         T* pr = Pr + (long long)row * ld;
         T* pi = CPLX ? Pi + (long long)row * ld : nullptr;
@@ -461,7 +465,7 @@ __device__ __noinline__ int block_lu(T* __restrict__ Pr, T* __restrict__ Pi, int
     for (int I = 0; I < ns; ++I) {
         T* PIr = Pr + I * ps;
         T* PIi = Pi + I * ps;
-        for (int e = tid; e < m * m; e += NT) {
+        for (int e = tid; e < m * m; e += RNT) {
             const int i = e / m, j = e - i * m;
             B0[i * LS + j] = PIr[(long long)i * ld + 3 * m + j];
             if (CPLX) B1[i * LS + j] = PIi[(long long)i * ld + 3 * m + j];
@@ -470,7 +474,7 @@ __device__ __noinline__ int block_lu(T* __restrict__ Pr, T* __restrict__ Pi, int
         lap(1);
         gj_inverse<CPLX, T>(L, m);
         lap(0);
-        for (int e = tid; e < m * m; e += NT) {
+        for (int e = tid; e < m * m; e += RNT) {
             const int i = e / m, j = e - i * m;
             PIr[(long long)i * ld + 3 * m + j] = B0[i * LS + j];
             if (CPLX) PIi[(long long)i * ld + 3 * m + j] = B1[i * LS + j];
@@ -486,7 +490,7 @@ __device__ __noinline__ int block_lu(T* __restrict__ Pr, T* __restrict__ Pi, int
                 T* Br = lds_block<T>(L, CPLX ? 2 * kb : kb);
                 T* Bi = CPLX ? lds_block<T>(L, 2 * kb + 1) : nullptr;
                 const int col = (4 + kb0 + kb) * m;
-                for (int e = tid; e < m * m; e += NT) {
+                for (int e = tid; e < m * m; e += RNT) {
                     const int i = e / m, j = e - i * m;
                     Br[i * LS + j] = PIr[(long long)i * ld + col + j];
                     if (CPLX) Bi[i * LS + j] = PIi[(long long)i * ld + col + j];
@@ -504,8 +508,8 @@ __device__ __noinline__ int block_lu(T* __restrict__ Pr, T* __restrict__ Pi, int
 }
 
 // Single right-hand side solve with the block factors, latency-shaped: each
-// thread (row i = tid / 4, part q = tid % 4) loads a block-row's factor
-// entries block by block (16 per thread, unrolled and predicated), issuing the
+// thread (row i = tid / 8, part q = tid % 8) loads a block-row's factor
+// entries block by block (8 per thread, unrolled and predicated), issuing the
 // next block's loads before contracting the current one, so a block-row step
 // waits for about one memory latency; the last solution blocks live in an LDS
 // ring (4 x 64, re and im) instead of being re-read from memory. ring: LDS of
@@ -516,13 +520,14 @@ __device__ __noinline__ void block_solve1(const T* __restrict__ Pr, const T* __r
                                           double* __restrict__ xr, double* __restrict__ xi, double* ring) {
     const int ld = 7 * m;
     const long long ps = (long long)m * ld;
-    const int tid = threadIdx.x, i = tid >> 2, q = tid & 3;
+    constexpr int TPR = RNT / MAX_BLOCK;  // threads per row
+    constexpr int U = MAX_BLOCK / TPR;    // columns per thread per block
+    const int tid = threadIdx.x, i = tid / TPR, q = tid % TPR;
     const bool row = i < m;
     double* rr = ring;
     double* ri = ring + 4 * 64;
     double* tr = ring + 8 * 64;
     double* ti = ring + 9 * 64;
-    constexpr int U = MAX_BLOCK / 4;
     double a0[U], b0[U], a1[U], b1[U];
     const T* pr = nullptr;
     const T* pi = nullptr;
@@ -530,7 +535,7 @@ __device__ __noinline__ void block_solve1(const T* __restrict__ Pr, const T* __r
     auto load = [&](int col, bool ok, double* a, double* b) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int k = q + 4 * u;
+            const int k = q + TPR * u;
             const bool v = ok && row && k < m;
             a[u] = v ? double(pr[col + k]) : 0.0;
             b[u] = (CPLX && v) ? double(pi[col + k]) : 0.0;
@@ -541,7 +546,7 @@ __device__ __noinline__ void block_solve1(const T* __restrict__ Pr, const T* __r
                         double& ai) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int k = q + 4 * u;
+            const int k = q + TPR * u;
             if (k < m) {
                 if (CPLX) {
                     ar += a[u] * yr[k] - b[u] * yi[k];
@@ -553,11 +558,10 @@ __device__ __noinline__ void block_solve1(const T* __restrict__ Pr, const T* __r
         }
     };
     auto reduce4 = [&](double& ar, double& ai) {
-        ar += __shfl_xor(ar, 1, 64);
-        ar += __shfl_xor(ar, 2, 64);
-        if (CPLX) {
-            ai += __shfl_xor(ai, 1, 64);
-            ai += __shfl_xor(ai, 2, 64);
+#pragma unroll
+        for (int o = 1; o < TPR; o <<= 1) {
+            ar += __shfl_xor(ar, o, 64);
+            if (CPLX) ai += __shfl_xor(ai, o, 64);
         }
     };
     // forward: y_I = b_I - sum_{d=0..2} L_{I, I-3+d} y_{I-3+d} (ring slot of block J: J & 3)

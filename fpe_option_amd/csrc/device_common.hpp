@@ -63,17 +63,20 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
-// Workgroup sum with a fixed reduction tree (deterministic). red: LDS[NWAVE].
-__device__ __forceinline__ double block_sum(double v, double* red) {
+// Workgroup sum over NW waves with a fixed reduction tree (deterministic). red: LDS[NW].
+template <int NW>
+__device__ __forceinline__ double block_sum_w(double v, double* red) {
     v = wave_sum(v);
     __syncthreads();
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
     double s = 0.0;
 #pragma unroll
-    for (int w = 0; w < NWAVE; ++w) s += red[w];
+    for (int w = 0; w < NW; ++w) s += red[w];
     return s;
 }
+
+__device__ __forceinline__ double block_sum(double v, double* red) { return block_sum_w<NWAVE>(v, red); }
 
 // K independent workgroup sums at once. red: LDS[K*NWAVE].
 template <int K>
@@ -95,15 +98,18 @@ __device__ __forceinline__ void block_sum_n(double* v, double* red) {
     }
 }
 
-__device__ __forceinline__ double block_max(double v, double* red) {
+template <int NW>
+__device__ __forceinline__ double block_max_w(double v, double* red) {
     v = wave_max(v);
     __syncthreads();
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
     double s = red[0];
 #pragma unroll
-    for (int w = 1; w < NWAVE; ++w) s = fmax(s, red[w]);
+    for (int w = 1; w < NW; ++w) s = fmax(s, red[w]);
     return s;
 }
+
+__device__ __forceinline__ double block_max(double v, double* red) { return block_max_w<NWAVE>(v, red); }
 
 }  // namespace fpe

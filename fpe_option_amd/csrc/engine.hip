@@ -234,7 +234,7 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord(G.ev[2], st));
     if (!opt.stop_after_ic) {
-        hipLaunchKernelGGL(k_radau, dim3(n_slots), dim3(NT), 0, st, dI, n, G.queue.as<int>(), G.st.as<double>(),
+        hipLaunchKernelGGL(k_radau, dim3(n_slots), dim3(RNT), 0, st, dI, n, G.queue.as<int>(), G.st.as<double>(),
                            G.vc.as<double>(), G.ws.as<double>(), G.wi.as<int>(), L, G.hs.as<double>(),
                            G.out.as<InstOut>());
         HIP_OK(hipGetLastError());
