@@ -78,22 +78,22 @@ __device__ __forceinline__ double block_sum_w(double v, double* red) {
 
 __device__ __forceinline__ double block_sum(double v, double* red) { return block_sum_w<NWAVE>(v, red); }
 
-// K independent workgroup sums at once. red: LDS[K*NWAVE].
-template <int K>
+// K independent workgroup sums at once over NW waves. red: LDS[K*NW].
+template <int K, int NW = NWAVE>
 __device__ __forceinline__ void block_sum_n(double* v, double* red) {
 #pragma unroll
     for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
     __syncthreads();
     if ((threadIdx.x & 63) == 0) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) red[k * NWAVE + (threadIdx.x >> 6)] = v[k];
+        for (int k = 0; k < K; ++k) red[k * NW + (threadIdx.x >> 6)] = v[k];
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         double s = 0.0;
 #pragma unroll
-        for (int w = 0; w < NWAVE; ++w) s += red[k * NWAVE + w];
+        for (int w = 0; w < NW; ++w) s += red[k * NW + w];
         v[k] = s;
     }
 }

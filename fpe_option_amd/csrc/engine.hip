@@ -229,7 +229,7 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
     hipLaunchKernelGGL(k_ic_rhs, dim3(n), dim3(NT), 0, st, dI, G.qd.as<double>(), G.bv.as<double>(),
                        G.bc.as<int>(), G.vc.as<double>());
     hipLaunchKernelGGL(k_eig, dim3(n, 2), dim3(NT), 0, st, dI, G.fc.as<double>(), G.eg.as<double>());
-    hipLaunchKernelGGL(k_admm, dim3(n), dim3(NT), ADMM_LDS_MAX, st, dI, G.st.as<double>(), G.eg.as<double>(),
+    hipLaunchKernelGGL(k_admm, dim3(n), dim3(ANT), ADMM_LDS_MAX, st, dI, G.st.as<double>(), G.eg.as<double>(),
                        G.vc.as<double>(), G.out.as<InstOut>());
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord(G.ev[2], st));

@@ -188,29 +188,43 @@ __global__ void __launch_bounds__(NT) k_eig(const Inst* __restrict__ inst, const
         }
 }
 
-// C = A B with a K-loop (operands typically in LDS), v_mfma_f64_16x16x4f64;
-// C(r, c, value) receives each output once.
-template <class FA, class FB, class FC>
+// C = A B (N <= 64) with a K-loop, v_mfma_f64_16x16x4f64; operands typically in
+// LDS. A wave owns a 16-row strip across all (at most 4) column tiles: each
+// k-step loads one A fragment and feeds four independent accumulator chains,
+// so an operand read from memory (the s eigenvectors when they do not fit in
+// LDS) is waited for once per k-step, not once per tile. The k order matches
+// a per-tile loop, so results are identical. C(r, c, value) receives each
+// output once. NW: waves in the workgroup.
+template <int NW, class FA, class FB, class FC>
 __device__ __forceinline__ void mfma_gemm_k(int M, int N, int K, FA A, FB B, FC C) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int tm = (M + 15) >> 4, tn = (N + 15) >> 4;
     const int l16 = lane & 15, l4 = lane >> 4;
-    for (int t = wave; t < tm * tn; t += NWAVE) {
-        const int r0 = (t / tn) << 4, c0 = (t % tn) << 4;
-        const int ra = r0 + l16, cb = c0 + l16;
-        d4 acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 4
+    for (int t = wave; t < tm; t += NW) {
+        const int r0 = t << 4, ra = r0 + l16;
+        d4 acc[4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+#pragma unroll 2
         for (int k0 = 0; k0 < K; k0 += 4) {
             const int k = k0 + l4;
             const double a = (ra < M && k < K) ? A(ra, k) : 0.0;
-            const double b = (cb < N && k < K) ? B(k, cb) : 0.0;
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
-        }
-        if (cb < N) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int r = r0 + l4 + 4 * j;
-                if (r < M) C(r, cb, acc[j]);
+            for (int u = 0; u < 4; ++u) {
+                if (u < tn) {
+                    const int cb = (u << 4) + l16;
+                    const double b = (cb < N && k < K) ? B(k, cb) : 0.0;
+                    acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[u], 0, 0, 0);
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int cb = (u << 4) + l16;
+            if (u < tn && cb < N) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int r = r0 + l4 + 4 * j;
+                    if (r < M) C(r, cb, acc[u][j]);
+                }
             }
         }
     }
@@ -228,7 +242,10 @@ constexpr long long ADMM_LDS_MAX = 150 * 1024;
 // (initial_cond.mojo:119-129). The four products per iteration run on MFMA
 // with Qv and the two ns x nv work matrices in LDS (Qs too when it fits).
 // grid: n_inst; dynamic LDS: ADMM_LDS_MAX.
-__global__ void __launch_bounds__(NT) k_admm(const Inst* __restrict__ inst, const double* __restrict__ ST,
+constexpr int ANT = 512;  // k_admm workgroup: 8 waves, 2 per SIMD
+constexpr int ANW = ANT / 64;
+
+__global__ void __launch_bounds__(ANT) k_admm(const Inst* __restrict__ inst, const double* __restrict__ ST,
                                              const double* __restrict__ EG, double* __restrict__ VC,
                                              InstOut* __restrict__ out) {
     extern __shared__ double dyn[];
@@ -247,7 +264,7 @@ __global__ void __launch_bounds__(NT) k_admm(const Inst* __restrict__ inst, cons
     const double* Ms = ST + in.o_st;
     const double rho = 0.1, sigma = 1e-6, lam = 1e-6, eps_abs = 1e-8, eps_rel = 1e-8;
     const double alpha = lam + sigma + rho;
-    __shared__ double red[5 * NWAVE];
+    __shared__ double red[5 * ANW];
     // LDS: Qv (nv x nv+1), T and R (ns x nv+1), Qs (ns x ns+1) when it fits
     const int lv = nv + 1, lsd = ns + 1;
     double* sQv = dyn;
@@ -255,12 +272,12 @@ __global__ void __launch_bounds__(NT) k_admm(const Inst* __restrict__ inst, cons
     double* sR = sT + ns * lv;
     const bool qs_lds = admm_lds_bytes(ns, nv, true) <= ADMM_LDS_MAX;
     double* sQs = sR + ns * lv;
-    for (int e = threadIdx.x; e < nv * nv; e += NT) sQv[(e / nv) * lv + e % nv] = E.Qv[e];
+    for (int e = threadIdx.x; e < nv * nv; e += ANT) sQv[(e / nv) * lv + e % nv] = E.Qv[e];
     if (qs_lds)
-        for (int e = threadIdx.x; e < ns * ns; e += NT) sQs[(e / ns) * lsd + e % ns] = E.Qs[e];
+        for (int e = threadIdx.x; e < ns * ns; e += ANT) sQs[(e / ns) * lsd + e % ns] = E.Qs[e];
     auto Qs = [&](int I, int a) -> double { return qs_lds ? sQs[I * lsd + a] : E.Qs[(long long)I * ns + a]; };
     // q = A^T b~ (A symmetric)
-    for (int r = threadIdx.x; r < N; r += NT) {
+    for (int r = threadIdx.x; r < N; r += ANT) {
         const int I = r / nv, i = r - I * nv;
         double acc = 0.0;
         for (int dJ = -3; dJ <= 3; ++dJ) {
@@ -279,14 +296,14 @@ __global__ void __launch_bounds__(NT) k_admm(const Inst* __restrict__ inst, cons
     int it = 0;
     const int max_iter = 50000;
     for (; it < max_iter;) {
-        for (int r = threadIdx.x; r < N; r += NT) sR[(r / nv) * lv + r % nv] = q[r] + rho * (z[r] - u[r]) + sigma * x[r];
+        for (int r = threadIdx.x; r < N; r += ANT) sR[(r / nv) * lv + r % nv] = q[r] + rho * (z[r] - u[r]) + sigma * x[r];
         __syncthreads();
         // T = Qs^T R
-        mfma_gemm_k(ns, nv, ns, [&](int a, int I) { return Qs(I, a); }, [&](int I, int i) { return sR[I * lv + i]; },
+        mfma_gemm_k<ANW>(ns, nv, ns, [&](int a, int I) { return Qs(I, a); }, [&](int I, int i) { return sR[I * lv + i]; },
                     [&](int a, int i, double v) { sT[a * lv + i] = v; });
         __syncthreads();
         // R = (T Qv) ./ (ls^2 lv^2 + alpha)
-        mfma_gemm_k(ns, nv, nv, [&](int a, int i) { return sT[a * lv + i]; },
+        mfma_gemm_k<ANW>(ns, nv, nv, [&](int a, int i) { return sT[a * lv + i]; },
                     [&](int i, int b) { return sQv[i * lv + b]; },
                     [&](int a, int b, double v) {
                         const double la = E.ls[a] * E.ls[a], lb = E.lv[b] * E.lv[b];
@@ -294,12 +311,12 @@ __global__ void __launch_bounds__(NT) k_admm(const Inst* __restrict__ inst, cons
                     });
         __syncthreads();
         // T = Qs R
-        mfma_gemm_k(ns, nv, ns, [&](int I, int a) { return Qs(I, a); }, [&](int a, int b) { return sR[a * lv + b]; },
+        mfma_gemm_k<ANW>(ns, nv, ns, [&](int I, int a) { return Qs(I, a); }, [&](int a, int b) { return sR[a * lv + b]; },
                     [&](int I, int b, double v) { sT[I * lv + b] = v; });
         __syncthreads();
         // x = T Qv^T, then the z and u updates and the five norms
         double nr[5] = {0, 0, 0, 0, 0};
-        mfma_gemm_k(ns, nv, nv, [&](int I, int b) { return sT[I * lv + b]; },
+        mfma_gemm_k<ANW>(ns, nv, nv, [&](int I, int b) { return sT[I * lv + b]; },
                     [&](int b, int i) { return sQv[i * lv + b]; },
                     [&](int I, int i, double xv) {
                         const int e = I * nv + i;
@@ -317,7 +334,7 @@ __global__ void __launch_bounds__(NT) k_admm(const Inst* __restrict__ inst, cons
                         nr[3] += xv * xv;
                         nr[4] += un * un;
                     });
-        block_sum_n<5>(nr, red);
+        block_sum_n<5, ANW>(nr, red);
         ++it;
         const double ep = eps_abs + eps_rel * sqrt(fmax(nr[2], nr[3]));
         const double ed = eps_abs + eps_rel * rho * sqrt(nr[4]);
@@ -325,15 +342,15 @@ __global__ void __launch_bounds__(NT) k_admm(const Inst* __restrict__ inst, cons
     }
     __syncthreads();
     double md = 0.0;
-    for (int r = threadIdx.x; r < N; r += NT) {
+    for (int r = threadIdx.x; r < N; r += ANT) {
         double v = Di[r] * z[r];
         if (v < 0.0) v = 0.0;
         q0[r] = v;
         md += mw[r] * v;
     }
-    md = block_sum(md, red);
+    md = block_sum_w<ANW>(md, red);
     if (md > 0.0)
-        for (int r = threadIdx.x; r < N; r += NT) q0[r] = q0[r] / md;
+        for (int r = threadIdx.x; r < N; r += ANT) q0[r] = q0[r] / md;
     if (threadIdx.x == 0) out[blockIdx.x].n_admm = it;
 }
 
