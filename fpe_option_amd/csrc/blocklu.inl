@@ -19,17 +19,19 @@
 
 namespace fpe {
 
-// The Radau workgroup runs 8 waves (2 per SIMD), so one wave's memory or
-// barrier wait overlaps the other's work on the same SIMD.
-constexpr int RNT = 512;
+// Radau launch shape: 4-wave workgroups, two per CU (about 74 KB of LDS
+// each), so two independent solves share every SIMD and one's memory and
+// barrier waits overlap the other's work.
+constexpr int RNT = 256;
 constexpr int RNW = RNT / 64;
+constexpr int RADAU_WG_PER_CU = 2;
 constexpr int LS = MAX_BLOCK + 1;  // LDS row stride of an m x m block
 constexpr int NRG = MAX_BLOCK / RNW;  // register columns per lane in the Gauss-Jordan inverse
 
 struct LuLds {
-    // four m x m staging blocks: the pivot block and its inverse (0, 1 = re, im),
-    // unscramble scratch (2, 3), then the U blocks of the Schur update
-    double B[4][MAX_BLOCK * LS];
+    // two m x m staging blocks: the pivot block and its inverse (re, im), then
+    // the U blocks of the Schur update; idle during solves (solution ring)
+    double B[2][MAX_BLOCK * LS];
     int piv[MAX_BLOCK];   // pivot row of each elimination step
     int qinv[MAX_BLOCK];  // step whose pivot row is a given row
     double colr[2][64], coli[2][64];  // pivot column of a step (double-buffered)
@@ -141,16 +143,14 @@ __device__ __forceinline__ const T* lds_block(const LuLds& L, int k) { return re
 // pivot before updating its other columns, so the search overlaps the other
 // waves' updates. Diagonal-pivot in-place Gauss-Jordan on P A leaves
 // X[p_k][j] = inv(A)[k][p_j]; the last pass unscrambles inv(A)[a][b] = X[p_a][q_b]
-// with p_{q_b} = b. Result in natural order in (B[0], B[1]); B[2], B[3] are
-// scratch. Sets singular on a zero or non-finite pivot. Ends with a barrier.
+// with p_{q_b} = b in place. Result in natural order in (B[0], B[1]). Sets
+// singular on a zero or non-finite pivot. Ends with a barrier.
 template <bool CPLX, class T>
 __device__ void gj_inverse(LuLds& L, int m) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const bool lrow = lane < m;
     T* B0 = lds_block<T>(L, 0);
     T* B1 = lds_block<T>(L, 1);
-    T* B2 = lds_block<T>(L, 2);
-    T* B3 = lds_block<T>(L, 3);
     T xr[NRG], xi[NRG];
 #pragma unroll
     for (int r = 0; r < NRG; ++r) {
@@ -238,18 +238,35 @@ __device__ void gj_inverse(LuLds& L, int m) {
     for (int r = 0; r < NRG; ++r) {
         const int j = RNW * r + w;
         if (lrow && j < m) {
-            B2[lane * LS + j] = xr[r];
-            if (CPLX) B3[lane * LS + j] = xi[r];
+            B0[lane * LS + j] = xr[r];
+            if (CPLX) B1[lane * LS + j] = xi[r];
         }
     }
     __syncthreads();
     if (threadIdx.x < m) L.qinv[L.piv[threadIdx.x]] = threadIdx.x;
     __syncthreads();
-    for (int e = threadIdx.x; e < m * m; e += RNT) {
-        const int a = e / m, c = e - a * m;
-        const int src = L.piv[a] * LS + L.qinv[c];
-        B0[a * LS + c] = B2[src];
-        if (CPLX) B1[a * LS + c] = B3[src];
+    // unscramble in place through registers: gather, barrier, scatter
+    constexpr int GE = MAX_BLOCK * MAX_BLOCK / RNT;
+    T gr[GE], gi[GE];
+#pragma unroll
+    for (int g = 0; g < GE; ++g) {
+        const int e = threadIdx.x + g * RNT;
+        if (e < m * m) {
+            const int a = e / m, c = e - a * m;
+            const int src = L.piv[a] * LS + L.qinv[c];
+            gr[g] = B0[src];
+            gi[g] = CPLX ? B1[src] : T(0);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int g = 0; g < GE; ++g) {
+        const int e = threadIdx.x + g * RNT;
+        if (e < m * m) {
+            const int a = e / m, c = e - a * m;
+            B0[a * LS + c] = gr[g];
+            if (CPLX) B1[a * LS + c] = gi[g];
+        }
     }
     __syncthreads();
 }
@@ -323,7 +340,8 @@ __device__ void lower_multiply(T* __restrict__ Pr, T* __restrict__ Pi, long long
 }
 
 // A_JK -= L_JI U_IK for J = I+1..I+nU and K = I+1+kb0 .. I+kb0+nkb, with U_IK
-// staged in LDS (B[2 kb] re, B[2 kb + 1] im in complex arithmetic; B[kb] in real).
+// staged in LDS (B[0] re, B[1] im in complex arithmetic, one block per pass;
+// B[kb] in real).
 // L strips come from the panels into registers once per strip; the C tile of
 // the next MFMA chain is loaded before the chain so its latency overlaps it.
 template <bool CPLX, class T>
@@ -458,8 +476,8 @@ __device__ __noinline__ int block_lu(T* __restrict__ Pr, T* __restrict__ Pi, int
             mark = now;
         }
     };
-    // blocks per Schur pass: 3 U blocks fit the staging buffers in real arithmetic, 2 in complex
-    constexpr int KPASS = CPLX ? 2 : 3;
+    // U blocks per Schur pass: 2 fit the staging buffers in real arithmetic, 1 in complex
+    constexpr int KPASS = CPLX ? 1 : 2;
     T* B0 = lds_block<T>(L, 0);
     T* B1 = lds_block<T>(L, 1);
     for (int I = 0; I < ns; ++I) {

@@ -186,7 +186,7 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
     L.stride = (L.stride + 31) & ~31LL;
     L.istride = ((2 * max_nm) + 63) & ~63LL;
     L.has_complex = any_direct;
-    const int n_slots = std::min(n, std::max(1, g_dev.n_cu));
+    const int n_slots = std::min(n, std::max(1, RADAU_WG_PER_CU * g_dev.n_cu));
 
     std::vector<PriceReq> hr(reqs.size());
     for (size_t k = 0; k < reqs.size(); ++k) {

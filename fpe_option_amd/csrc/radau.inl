@@ -178,7 +178,7 @@ __device__ void stage_residual(const double* __restrict__ Ms, const double* __re
     }
 }
 
-__global__ void __launch_bounds__(RNT) k_radau(const Inst* __restrict__ inst, int n_inst, int* __restrict__ queue,
+__global__ void __launch_bounds__(RNT, RADAU_WG_PER_CU * RNW / 4) k_radau(const Inst* __restrict__ inst, int n_inst, int* __restrict__ queue,
                                               const double* __restrict__ ST, double* __restrict__ VC,
                                               double* __restrict__ WS, int* __restrict__ WI, SlotLayout L,
                                               double* __restrict__ HS, InstOut* __restrict__ out) {
