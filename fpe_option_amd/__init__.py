@@ -12,3 +12,4 @@ from .api import (  # noqa: F401
 
 __version__ = "0.1.0"
 _logger = logging.getLogger("fpe_engine")
+from .calibration import calibrate, calibrate_batch  # noqa: E402,F401
