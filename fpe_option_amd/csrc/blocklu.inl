@@ -286,6 +286,7 @@ template <bool CPLX, class T>
 __device__ void lower_multiply(T* __restrict__ Pr, T* __restrict__ Pi, long long ps, int ld, int I, int m, int nU,
                                const LuLds& L) {
     typedef typename Mf<T>::acc A4;
+    constexpr int TG = CPLX ? 2 : 4;  // column tiles per accumulator group
     const T* B0 = lds_block<T>(L, 0);
     const T* B1 = lds_block<T>(L, 1);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -313,27 +314,38 @@ __device__ void lower_multiply(T* __restrict__ Pr, T* __restrict__ Pi, long long
             int dJ;
             oc[j] = vc[j] ? stacked_row(ps, ld, I, m, r, dJ) + (3 - dJ) * m : 0;
         }
-        for (int u = 0; u < tn; ++u) {
-            const int c = (u << 4) + l16;
-            const bool cv = c < m;
-            A4 accr = {0, 0, 0, 0}, acci = {0, 0, 0, 0};
+        // TG column tiles of the strip at a time: independent accumulator chains
+        for (int ug = 0; ug < tn; ug += TG) {
+            A4 accr[TG], acci[TG];
+#pragma unroll
+            for (int u = 0; u < TG; ++u) accr[u] = A4{0, 0, 0, 0}, acci[u] = A4{0, 0, 0, 0};
 #pragma unroll
             for (int s = 0; s < MAX_BLOCK / 4; ++s) {
                 if (s < tk) {
                     const int k = 4 * s + l4;
-                    const bool ok = cv && k < m;
-                    const T br = ok ? B0[k * LS + c] : T(0);
-                    const T bi = (CPLX && ok) ? B1[k * LS + c] : T(0);
-                    mfma_c(accr, acci, ar[s], ai[s], br, bi, CPLX);
+#pragma unroll
+                    for (int u = 0; u < TG; ++u) {
+                        if (ug + u < tn) {
+                            const int c = ((ug + u) << 4) + l16;
+                            const bool ok = c < m && k < m;
+                            const T br = ok ? B0[k * LS + c] : T(0);
+                            const T bi = (CPLX && ok) ? B1[k * LS + c] : T(0);
+                            mfma_c(accr[u], acci[u], ar[s], ai[s], br, bi, CPLX);
+                        }
+                    }
                 }
             }
-            if (cv) {
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    if (vc[j]) {
-                        Pr[oc[j] + c] = accr[j];
-                        if (CPLX) Pi[oc[j] + c] = acci[j];
-                    }
+            for (int u = 0; u < TG; ++u) {
+                const int c = ((ug + u) << 4) + l16;
+                if (ug + u < tn && c < m) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        if (vc[j]) {
+                            Pr[oc[j] + c] = accr[u][j];
+                            if (CPLX) Pi[oc[j] + c] = acci[u][j];
+                        }
+                }
             }
         }
     }
@@ -342,12 +354,14 @@ __device__ void lower_multiply(T* __restrict__ Pr, T* __restrict__ Pi, long long
 // A_JK -= L_JI U_IK for J = I+1..I+nU and K = I+1+kb0 .. I+kb0+nkb, with U_IK
 // staged in LDS (B[0] re, B[1] im in complex arithmetic, one block per pass;
 // B[kb] in real).
-// L strips come from the panels into registers once per strip; the C tile of
-// the next MFMA chain is loaded before the chain so its latency overlaps it.
+// L strips come from the panels into registers once per strip; the column
+// tiles of a K block run in groups (C tiles loaded up front, one independent
+// accumulator chain per tile).
 template <bool CPLX, class T>
 __device__ void schur_pass(T* __restrict__ Pr, T* __restrict__ Pi, long long ps, int ld, int I, int m, int nU,
                            int kb0, int nkb, const LuLds& L) {
     typedef typename Mf<T>::acc A4;
+    constexpr int TG = CPLX ? 2 : 4;  // column tiles per accumulator group
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, l4 = lane >> 4;
     const int M = nU * m, tm = (M + 15) >> 4, tk = (m + 3) >> 2, tn = (m + 15) >> 4;
@@ -378,35 +392,52 @@ __device__ void schur_pass(T* __restrict__ Pr, T* __restrict__ Pi, long long ps,
             const T* Br = lds_block<T>(L, CPLX ? 2 * kb : kb);
             const T* Bi = lds_block<T>(L, CPLX ? 2 * kb + 1 : kb);
             const int kk = kb0 + kb;  // K = I + 1 + kk; column block d = kk + 4 - dJ in panel I + dJ
-            for (int u = 0; u < tn; ++u) {
-                const int c = (u << 4) + l16;
-                const bool cv = c < m;
-                T cr[4], ci[4];
-                long long o[4];
+            // TG column tiles of the K block at a time: C tiles loaded up front,
+            // independent accumulator chains per tile
+            for (int ug = 0; ug < tn; ug += TG) {
+                T cr[TG][4], ci[TG][4];
+                A4 accr[TG], acci[TG];
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    o[j] = oc[j] + (long long)(kk + 4 - dj[j]) * m + c;
-                    const bool ok = vc[j] && cv;
-                    cr[j] = ok ? Pr[o[j]] : T(0);
-                    ci[j] = (CPLX && ok) ? Pi[o[j]] : T(0);
+                for (int u = 0; u < TG; ++u) {
+                    const int c = ((ug + u) << 4) + l16;
+                    accr[u] = A4{0, 0, 0, 0}, acci[u] = A4{0, 0, 0, 0};
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const bool ok = ug + u < tn && vc[j] && c < m;
+                        const long long o = oc[j] + (long long)(kk + 4 - dj[j]) * m + c;
+                        cr[u][j] = ok ? Pr[o] : T(0);
+                        ci[u][j] = (CPLX && ok) ? Pi[o] : T(0);
+                    }
                 }
-                A4 accr = {0, 0, 0, 0}, acci = {0, 0, 0, 0};
 #pragma unroll
                 for (int s = 0; s < MAX_BLOCK / 4; ++s) {
                     if (s < tk) {
                         const int k = 4 * s + l4;
-                        const bool ok = cv && k < m;
-                        const T br = ok ? Br[k * LS + c] : T(0);
-                        const T bi = (CPLX && ok) ? Bi[k * LS + c] : T(0);
-                        mfma_c(accr, acci, ar[s], ai[s], br, bi, CPLX);
+#pragma unroll
+                        for (int u = 0; u < TG; ++u) {
+                            if (ug + u < tn) {
+                                const int c = ((ug + u) << 4) + l16;
+                                const bool ok = c < m && k < m;
+                                const T br = ok ? Br[k * LS + c] : T(0);
+                                const T bi = (CPLX && ok) ? Bi[k * LS + c] : T(0);
+                                mfma_c(accr[u], acci[u], ar[s], ai[s], br, bi, CPLX);
+                            }
+                        }
                     }
                 }
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    if (vc[j] && cv) {
-                        Pr[o[j]] = cr[j] - accr[j];
-                        if (CPLX) Pi[o[j]] = ci[j] - acci[j];
+                for (int u = 0; u < TG; ++u) {
+                    const int c = ((ug + u) << 4) + l16;
+                    if (ug + u < tn && c < m) {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            if (vc[j]) {
+                                const long long o = oc[j] + (long long)(kk + 4 - dj[j]) * m + c;
+                                Pr[o] = cr[u][j] - accr[u][j];
+                                if (CPLX) Pi[o] = ci[u][j] - acci[u][j];
+                            }
                     }
+                }
             }
         }
     }

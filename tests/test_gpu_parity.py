@@ -206,3 +206,12 @@ def test_native_module_drives_reference_shaped_wrapper(engine, oracle):
     assert len(p) == 2 and np.all(np.isfinite(p))
     r = native.price(dict(HESTON, K=[60.0], option_type=8, n_s=16, n_v=16))
     assert set(r) == {"prices", "deltas", "gammas", "vegas"} and len(r["prices"]) == 1
+
+
+@pytest.mark.parametrize("ot,B", [(6, 75.0), (7, 70.0), (4, 75.0), (5, 70.0), (1, 50.0)])
+def test_reference_mode_up_barriers_and_down_in_put(engine, oracle, ot, B):
+    # up types truncate the domain at s_max = B with a Dirichlet right end and a
+    # Neumann left end (src/server/option_types.mojo:34-70); in-types by parity
+    K = [55.0, 60.0, 65.0]
+    kw = dict(HESTON, n_s=20, n_v=16, num_insert=50, s_max=180.0, option_type=ot, barrier=B, complex_mode=1)
+    _parity(K, kw, oracle, engine)
