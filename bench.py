@@ -190,6 +190,12 @@ def main():
                                                    "ms_total")},
         "solver": {k: int(st[k]) for k in ("n_solves", "n_accepted", "n_rejected", "n_lu", "n_newton",
                                            "n_err_solves", "n_inner_iters", "n_admm_iters")},
+        # Radau workgroups: busy fraction over the kernel's duration, and where their time goes
+        "radau_slots": {"slots": int(st["n_radau_slots"]),
+                        "busy_frac": round(st["wg_s_total"] / max(1, st["n_radau_slots"]) / radau_s, 4),
+                        "wg_s": {k[5:]: round(st[k], 1) for k in (
+                            "wg_s_assemble", "wg_s_lu", "wg_s_solve", "wg_s_spmv", "wg_s_other", "wg_s_total",
+                            "wg_s_lu_inverse", "wg_s_lu_stage", "wg_s_lu_lmul", "wg_s_lu_schur")}},
         "roofline": {"bound": "mfma", "kernel": "k_radau", "achieved": round(achieved_tf, 4),
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved_tf / FP64_PEAK_TFLOPS, 5),
                      "traffic": profiled_traffic(args.n, args.mode),

@@ -67,7 +67,7 @@ class FpeBatchStats(C.Structure):
         (n, C.c_double) for n in ("ms_setup", "ms_assemble", "ms_ic", "ms_radau", "ms_payoff", "ms_total",
                                   "alg_flops", "alg_bytes", "wg_s_assemble", "wg_s_lu", "wg_s_solve",
                                   "wg_s_spmv", "wg_s_other", "wg_s_total", "wg_s_lu_inverse", "wg_s_lu_stage",
-                                  "wg_s_lu_lmul", "wg_s_lu_schur")] + [("n_refine", C.c_int64)]
+                                  "wg_s_lu_lmul", "wg_s_lu_schur")] + [("n_refine", C.c_int64), ("n_radau_slots", C.c_int64)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

@@ -386,6 +386,7 @@ int32_t fpe_batch_price(const FpeBatchOption* opts, int32_t n, double* prices, i
         }
         stats->ms_setup = bt.setup, stats->ms_assemble = bt.assemble, stats->ms_ic = bt.ic;
         stats->ms_radau = bt.radau, stats->ms_payoff = bt.payoff, stats->ms_total = bt.total;
+        stats->n_radau_slots = bt.n_slots;
     }
     return 0;
 }
@@ -477,6 +478,7 @@ int32_t fpe_batch_terminal_states(const FpeBatchOption* opts, int32_t n, double*
         }
         stats->ms_setup = bt.setup, stats->ms_assemble = bt.assemble, stats->ms_ic = bt.ic;
         stats->ms_radau = bt.radau, stats->ms_payoff = bt.payoff, stats->ms_total = bt.total;
+        stats->n_radau_slots = bt.n_slots;
     }
     return 0;
 }

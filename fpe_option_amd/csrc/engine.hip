@@ -53,7 +53,7 @@ struct DeviceState {
     int n_cu = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[8];
-    DBuf inst, out, kn, qd, bv, bc, fc, st, vc, eg, hs, wm, ow, ws, wi, req, part, queue, pdf;
+    DBuf inst, out, kn, qd, bv, bc, fc, st, vc, eg, hs, wm, ow, ws, wi, req, part, queue, order, pdf;
 };
 
 std::mutex g_mu;
@@ -187,6 +187,16 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
     L.istride = ((2 * max_nm) + 63) & ~63LL;
     L.has_complex = any_direct;
     const int n_slots = std::min(n, std::max(1, RADAU_WG_PER_CU * g_dev.n_cu));
+    times.n_slots = n_slots;
+    // Radau work queue order: largest estimated factorisation cost (n_sb n_vb^3) first,
+    // so the slots finish together instead of waiting on a late large instance.
+    std::vector<int> order(n);
+    for (int s = 0; s < n; ++s) order[s] = s;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        const double ca = double(hi[a].nsb) * hi[a].nvb * hi[a].nvb * hi[a].nvb;
+        const double cb = double(hi[b].nsb) * hi[b].nvb * hi[b].nvb * hi[b].nvb;
+        return ca > cb;
+    });
 
     std::vector<PriceReq> hr(reqs.size());
     for (size_t k = 0; k < reqs.size(); ++k) {
@@ -199,7 +209,7 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
     DeviceState& G = g_dev;
     hipStream_t st = G.stream;
     if (!upload(G.inst, hi, err) || !upload(G.kn, kn, err) || !upload(G.qd, qd, err) || !upload(G.ow, ow, err) ||
-        !upload(G.req, hr, err))
+        !upload(G.req, hr, err) || !upload(G.order, order, err))
         return false;
     if (!G.out.ensure(sizeof(InstOut) * n, err) || !G.bv.ensure(sizeof(double) * o_bv, err) ||
         !G.bc.ensure(sizeof(int) * (o_bv / 8 + 1), err) || !G.fc.ensure(sizeof(double) * o_fc, err) ||
@@ -234,7 +244,7 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord(G.ev[2], st));
     if (!opt.stop_after_ic) {
-        hipLaunchKernelGGL(k_radau, dim3(n_slots), dim3(RNT), 0, st, dI, n, G.queue.as<int>(), G.st.as<double>(),
+        hipLaunchKernelGGL(k_radau, dim3(n_slots), dim3(RNT), 0, st, dI, n, G.queue.as<int>(), G.order.as<int>(), G.st.as<double>(),
                            G.vc.as<double>(), G.ws.as<double>(), G.wi.as<int>(), L, G.hs.as<double>(),
                            G.out.as<InstOut>());
         HIP_OK(hipGetLastError());

@@ -24,6 +24,7 @@ struct SolveResult {
 
 struct BatchTimes {
     double setup = 0, assemble = 0, ic = 0, radau = 0, payoff = 0, total = 0;
+    int n_slots = 0;  // Radau workgroups launched
 };
 
 struct Request {

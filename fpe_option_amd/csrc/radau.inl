@@ -178,7 +178,7 @@ __device__ void stage_residual(const double* __restrict__ Ms, const double* __re
     }
 }
 
-__global__ void __launch_bounds__(RNT, RADAU_WG_PER_CU * RNW / 4) k_radau(const Inst* __restrict__ inst, int n_inst, int* __restrict__ queue,
+__global__ void __launch_bounds__(RNT, RADAU_WG_PER_CU * RNW / 4) k_radau(const Inst* __restrict__ inst, int n_inst, int* __restrict__ queue, const int* __restrict__ order,
                                               const double* __restrict__ ST, double* __restrict__ VC,
                                               double* __restrict__ WS, int* __restrict__ WI, SlotLayout L,
                                               double* __restrict__ HS, InstOut* __restrict__ out) {
@@ -189,9 +189,10 @@ __global__ void __launch_bounds__(RNT, RADAU_WG_PER_CU * RNW / 4) k_radau(const 
     for (;;) {
         if (tid == 0) S.inst = atomicAdd(queue, 1);
         __syncthreads();
-        const int id = S.inst;
+        const int q = S.inst;
         __syncthreads();
-        if (id >= n_inst) break;
+        if (q >= n_inst) break;
+        const int id = order[q];  // instances are dealt largest first
         const Inst& in = inst[id];
         const int ns = in.nsb, nv = in.nvb, N = ns * nv, m = nv;
         const double* Ms = ST + in.o_st;

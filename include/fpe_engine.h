@@ -179,6 +179,9 @@ typedef struct FpeBatchStats {
     double wg_s_lu_inverse, wg_s_lu_stage, wg_s_lu_lmul, wg_s_lu_schur;
     /* FP64 refinement rounds of the mixed-precision stage solves (FPE_PREC_MIXED). */
     int64_t n_refine;
+    /* Radau workgroups launched (two per compute unit, at most one per solve);
+     * wg_s_total / (n_radau_slots * ms_radau / 1e3) is their busy fraction. */
+    int64_t n_radau_slots;
 } FpeBatchStats;
 
 /** CSR matrix (row-major, sorted columns). */
