@@ -71,6 +71,8 @@ bool dev_init(std::string& err) {
     HIP_OK(hipDeviceGetAttribute(&g_dev.n_cu, hipDeviceAttributeMultiprocessorCount, 0));
     HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_admm), hipFuncAttributeMaxDynamicSharedMemorySize,
                                int(ADMM_LDS_MAX)));
+    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_eig), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               int(EIG_LDS_S)));
     HIP_OK(hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking));
     for (auto& e : g_dev.ev) HIP_OK(hipEventCreate(&e));
     g_dev.init = true;
@@ -236,10 +238,13 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
                        G.vc.as<double>());
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord(G.ev[1], st));
-    hipLaunchKernelGGL(k_ic_rhs, dim3(n), dim3(NT), 0, st, dI, G.qd.as<double>(), G.bv.as<double>(),
+    hipLaunchKernelGGL(k_ic_rhs, dim3(n), dim3(NT), 0, st, dI, G.order.as<int>(), G.qd.as<double>(), G.bv.as<double>(),
                        G.bc.as<int>(), G.vc.as<double>());
-    hipLaunchKernelGGL(k_eig, dim3(n, 2), dim3(NT), 0, st, dI, G.fc.as<double>(), G.eg.as<double>());
-    hipLaunchKernelGGL(k_admm, dim3(n), dim3(ANT), ADMM_LDS_MAX, st, dI, G.st.as<double>(), G.eg.as<double>(),
+    hipLaunchKernelGGL(k_eig, dim3(n), dim3(ENT), EIG_LDS_S, st, dI, G.order.as<int>(), 0, EIG_LDS_S,
+                       G.fc.as<double>(), G.eg.as<double>());
+    hipLaunchKernelGGL(k_eig, dim3(n), dim3(ENT), EIG_LDS_V, st, dI, G.order.as<int>(), 1, EIG_LDS_V,
+                       G.fc.as<double>(), G.eg.as<double>());
+    hipLaunchKernelGGL(k_admm, dim3(n), dim3(ANT), ADMM_LDS_MAX, st, dI, G.order.as<int>(), G.st.as<double>(), G.eg.as<double>(),
                        G.vc.as<double>(), G.out.as<InstOut>());
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord(G.ev[2], st));

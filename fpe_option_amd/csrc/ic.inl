@@ -30,10 +30,10 @@ struct EigView {
 
 // Right-hand side b~ = D (Bs (x) Bv)^T W delta and the basis integrals
 // m = (Bs (x) Bv)^T W 1 (separable).  grid: n_inst.
-__global__ void __launch_bounds__(NT) k_ic_rhs(const Inst* __restrict__ inst, const double* __restrict__ QD,
+__global__ void __launch_bounds__(NT) k_ic_rhs(const Inst* __restrict__ inst, const int* __restrict__ order, const double* __restrict__ QD,
                                                const double* __restrict__ BV, const int* __restrict__ BC,
                                                double* __restrict__ VC) {
-    const Inst& in = inst[blockIdx.x];
+    const Inst& in = inst[order[blockIdx.x]];  // largest instances first
     const int ns = in.nsb, nv = in.nvb, N = ns * nv, nqs = in.nqs, nqv = in.nqv;
     const double* sw = QD + in.o_sq + nqs;
     const double* sph = QD + in.o_sq + 2 * nqs;
@@ -94,98 +94,132 @@ __global__ void __launch_bounds__(NT) k_ic_rhs(const Inst* __restrict__ inst, co
 }
 
 // Jacobi eigendecomposition of As (axis 0) or Av (axis 1) with parallel
-// round-robin ordering.  grid: (n_inst, 2).
-__global__ void __launch_bounds__(NT) k_eig(const Inst* __restrict__ inst, const double* __restrict__ FC,
-                                            double* __restrict__ EG) {
-    const Inst& in = inst[blockIdx.x];
-    const int ax = blockIdx.y;
+// round-robin ordering. One launch per axis: grid n_inst, dynamic LDS
+// lds_bytes. A (row stride n + 1) lives in LDS when it fits, else in the
+// instance's As/Av scratch; the eigenvector matrix is kept transposed
+// (Qt[p][row] = Q[row][p]) so that the column rotations Q <- Q J walk
+// contiguous rows: in LDS when A and Qt both fit, else in the Qs region
+// (transposed in place at the end). The arithmetic is the same wherever the
+// operands live.
+constexpr int ENT = 512;  // k_eig workgroup: 8 waves
+constexpr int ENW = ENT / 64;
+constexpr long long EIG_LDS_S = 150 * 1024;                          // axis 0 launch
+constexpr long long EIG_LDS_V = 2LL * MAX_BLOCK * (MAX_BLOCK + 1) * 8;  // axis 1: A and Qt of n_vb <= 64
+
+__global__ void __launch_bounds__(ENT) k_eig(const Inst* __restrict__ inst, const int* __restrict__ order, int ax,
+                                             long long lds_bytes, const double* __restrict__ FC,
+                                             double* __restrict__ EG) {
+    extern __shared__ double dyn[];
+    const Inst& in = inst[order[blockIdx.x]];  // largest instances first
     const int ns = in.nsb, nv = in.nvb;
     EigView E(EG + in.o_eig, ns, nv);
     const int n = ax == 0 ? ns : nv;
     const double* F = FC + (ax == 0 ? in.o_fs : in.o_fv);  // factor 0 = mass
-    double* A = ax == 0 ? E.As : E.Av;
-    double* Q = ax == 0 ? E.Qs : E.Qv;
+    const bool a_lds = 8LL * n * (n + 1) <= lds_bytes;
+    const bool q_lds = a_lds && 16LL * n * (n + 1) <= lds_bytes;
+    const int lda = a_lds ? n + 1 : n, ldq = q_lds ? n + 1 : n;
+    double* A = a_lds ? dyn : (ax == 0 ? E.As : E.Av);
+    double* Qt = q_lds ? dyn + (long long)n * (n + 1) : (ax == 0 ? E.Qs : E.QvT);
     double* lam = ax == 0 ? E.ls : E.lv;
-    __shared__ double cc[160], ss[160], red[NWAVE];
+    __shared__ double cc[160], ss[160];
     __shared__ int pp[160], qq[160];
-    for (int e = threadIdx.x; e < n * n; e += NT) {
+    __shared__ double red2[2 * ENW];
+    for (int e = threadIdx.x; e < n * n; e += ENT) {
         const int i = e / n, j = e - i * n;
         double v = 0.0;
         if (j - i >= -3 && j - i <= 3) {
             const double mii = F[(long long)i * 7 + 3], mjj = F[(long long)j * 7 + 3];
             v = F[(long long)i * 7 + (j - i + 3)] / sqrt(mii * mjj);
         }
-        A[e] = v;
-        Q[e] = (i == j) ? 1.0 : 0.0;
+        A[(long long)i * lda + j] = v;
+        Qt[(long long)i * ldq + j] = (i == j) ? 1.0 : 0.0;
     }
     __syncthreads();
     const int np = n + (n & 1);
     const int npair = np / 2;
     for (int sweep = 0; sweep < 30; ++sweep) {
         double off = 0.0, dg = 0.0;
-        for (int e = threadIdx.x; e < n * n; e += NT) {
+        for (int e = threadIdx.x; e < n * n; e += ENT) {
             const int i = e / n, j = e - i * n;
-            const double v = A[e] * A[e];
+            const double a = A[(long long)i * lda + j];
+            const double v = a * a;
             if (i == j) dg += v; else off += v;
         }
         double v2[2] = {off, dg};
-        __shared__ double red2[2 * NWAVE];
-        block_sum_n<2>(v2, red2);
+        block_sum_n<2, ENW>(v2, red2);
         if (v2[0] <= 1e-32 * v2[1]) break;
         for (int step = 0; step < np - 1; ++step) {
-            for (int k = threadIdx.x; k < npair; k += NT) {
+            for (int k = threadIdx.x; k < npair; k += ENT) {
                 auto idx = [&](int a) { return a == 0 ? 0 : 1 + (a - 1 + step) % (np - 1); };
                 int p = idx(k), q = idx(np - 1 - k);
                 if (p > q) { int t = p; p = q; q = t; }
-                double c = 1.0, s = 0.0;
+                double c = 1.0, sn = 0.0;
                 if (q < n) {
-                    const double apq = A[(long long)p * n + q];
-                    const double app = A[(long long)p * n + p], aqq = A[(long long)q * n + q];
+                    const double apq = A[(long long)p * lda + q];
+                    const double app = A[(long long)p * lda + p], aqq = A[(long long)q * lda + q];
                     if (fabs(apq) > 1e-300 && fabs(apq) > 1e-18 * sqrt(fabs(app * aqq))) {
                         const double tau = (aqq - app) / (2.0 * apq);
                         const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
                         c = 1.0 / sqrt(1.0 + t * t);
-                        s = t * c;
+                        sn = t * c;
                     }
                 } else {
                     q = -1;
                 }
-                cc[k] = c, ss[k] = s, pp[k] = p, qq[k] = q;
+                cc[k] = c, ss[k] = sn, pp[k] = p, qq[k] = q;
             }
             __syncthreads();
-            // rows: A <- J^T A
-            for (int e = threadIdx.x; e < npair * n; e += NT) {
+            // rows: A <- J^T A, and Qt <- J^T Qt (= (Q J)^T)
+            for (int e = threadIdx.x; e < npair * n; e += ENT) {
                 const int k = e / n, col = e - k * n;
                 const int p = pp[k], q = qq[k];
                 if (q < 0 || ss[k] == 0.0) continue;
-                const double c = cc[k], s = ss[k];
-                const double ap = A[(long long)p * n + col], aq = A[(long long)q * n + col];
-                A[(long long)p * n + col] = c * ap - s * aq;
-                A[(long long)q * n + col] = s * ap + c * aq;
+                const double c = cc[k], sn = ss[k];
+                const double ap = A[(long long)p * lda + col], aq = A[(long long)q * lda + col];
+                A[(long long)p * lda + col] = c * ap - sn * aq;
+                A[(long long)q * lda + col] = sn * ap + c * aq;
+                const double bp = Qt[(long long)p * ldq + col], bq = Qt[(long long)q * ldq + col];
+                Qt[(long long)p * ldq + col] = c * bp - sn * bq;
+                Qt[(long long)q * ldq + col] = sn * bp + c * bq;
             }
             __syncthreads();
-            // columns: A <- A J, Q <- Q J
-            for (int e = threadIdx.x; e < npair * n; e += NT) {
+            // columns: A <- A J
+            for (int e = threadIdx.x; e < npair * n; e += ENT) {
                 const int k = e / n, row = e - k * n;
                 const int p = pp[k], q = qq[k];
                 if (q < 0 || ss[k] == 0.0) continue;
-                const double c = cc[k], s = ss[k];
-                double ap = A[(long long)row * n + p], aq = A[(long long)row * n + q];
-                A[(long long)row * n + p] = c * ap - s * aq;
-                A[(long long)row * n + q] = s * ap + c * aq;
-                ap = Q[(long long)row * n + p], aq = Q[(long long)row * n + q];
-                Q[(long long)row * n + p] = c * ap - s * aq;
-                Q[(long long)row * n + q] = s * ap + c * aq;
+                const double c = cc[k], sn = ss[k];
+                const double ap = A[(long long)row * lda + p], aq = A[(long long)row * lda + q];
+                A[(long long)row * lda + p] = c * ap - sn * aq;
+                A[(long long)row * lda + q] = sn * ap + c * aq;
             }
             __syncthreads();
         }
     }
-    for (int i = threadIdx.x; i < n; i += NT) lam[i] = A[(long long)i * n + i];
-    if (ax == 1)
-        for (int e = threadIdx.x; e < n * n; e += NT) {
-            const int i = e / n, j = e - i * n;
-            E.QvT[(long long)j * n + i] = Q[e];
+    for (int i = threadIdx.x; i < n; i += ENT) lam[i] = A[(long long)i * lda + i];
+    if (ax == 0) {
+        if (q_lds) {
+            for (int e = threadIdx.x; e < n * n; e += ENT) {
+                const int i = e / n, j = e - i * n;
+                E.Qs[e] = Qt[(long long)j * ldq + i];
+            }
+        } else {  // Qt occupies Qs: transpose in place, one thread per pair
+            for (int e = threadIdx.x; e < n * n; e += ENT) {
+                const int i = e / n, j = e - i * n;
+                if (i < j) {
+                    const double a = Qt[(long long)i * n + j], b = Qt[(long long)j * n + i];
+                    Qt[(long long)i * n + j] = b;
+                    Qt[(long long)j * n + i] = a;
+                }
+            }
         }
+    } else {
+        for (int e = threadIdx.x; e < n * n; e += ENT) {
+            const int i = e / n, j = e - i * n;
+            E.Qv[e] = Qt[(long long)j * ldq + i];
+            if (q_lds) E.QvT[e] = Qt[(long long)i * ldq + j];
+        }
+    }
 }
 
 // C = A B (N <= 64) with a K-loop, v_mfma_f64_16x16x4f64; operands typically in
@@ -245,18 +279,17 @@ constexpr long long ADMM_LDS_MAX = 150 * 1024;
 constexpr int ANT = 512;  // k_admm workgroup: 8 waves, 2 per SIMD
 constexpr int ANW = ANT / 64;
 
-__global__ void __launch_bounds__(ANT) k_admm(const Inst* __restrict__ inst, const double* __restrict__ ST,
+__global__ void __launch_bounds__(ANT) k_admm(const Inst* __restrict__ inst, const int* __restrict__ order, const double* __restrict__ ST,
                                              const double* __restrict__ EG, double* __restrict__ VC,
                                              InstOut* __restrict__ out) {
     extern __shared__ double dyn[];
-    const Inst& in = inst[blockIdx.x];
+    const Inst& in = inst[order[blockIdx.x]];  // largest instances first
     const int ns = in.nsb, nv = in.nvb, N = ns * nv;
     EigView E(const_cast<double*>(EG) + in.o_eig, ns, nv);
     double* base = VC + in.o_vec;
     const double* Di = base + (long long)V_DINV * N;
     const double* bt = base + (long long)V_BT * N;
     const double* mw = base + (long long)V_MW * N;
-    double* x = base + (long long)V_ADMM_X * N;
     double* z = base + (long long)V_ADMM_Z * N;
     double* u = base + (long long)V_ADMM_U * N;
     double* q = base + (long long)V_ADMM_Q * N;
@@ -290,14 +323,13 @@ __global__ void __launch_bounds__(ANT) k_admm(const Inst* __restrict__ inst, con
             }
         }
         q[r] = acc;
-        x[r] = 0.0, z[r] = 0.0, u[r] = 0.0;
+        z[r] = 0.0, u[r] = 0.0;
+        sR[I * lv + i] = acc;  // first right-hand side: x = z = u = 0
     }
     __syncthreads();
     int it = 0;
     const int max_iter = 50000;
     for (; it < max_iter;) {
-        for (int r = threadIdx.x; r < N; r += ANT) sR[(r / nv) * lv + r % nv] = q[r] + rho * (z[r] - u[r]) + sigma * x[r];
-        __syncthreads();
         // T = Qs^T R
         mfma_gemm_k<ANW>(ns, nv, ns, [&](int a, int I) { return Qs(I, a); }, [&](int I, int i) { return sR[I * lv + i]; },
                     [&](int a, int i, double v) { sT[a * lv + i] = v; });
@@ -314,19 +346,21 @@ __global__ void __launch_bounds__(ANT) k_admm(const Inst* __restrict__ inst, con
         mfma_gemm_k<ANW>(ns, nv, ns, [&](int I, int a) { return Qs(I, a); }, [&](int a, int b) { return sR[a * lv + b]; },
                     [&](int I, int b, double v) { sT[I * lv + b] = v; });
         __syncthreads();
-        // x = T Qv^T, then the z and u updates and the five norms
+        // x = T Qv^T, then the z and u updates, the five norms and the next
+        // right-hand side q + rho (z - u) + sigma x (R is free: GEMM 3 read it
+        // before the last barrier; x itself is never stored)
         double nr[5] = {0, 0, 0, 0, 0};
         mfma_gemm_k<ANW>(ns, nv, nv, [&](int I, int b) { return sT[I * lv + b]; },
                     [&](int b, int i) { return sQv[i * lv + b]; },
                     [&](int I, int i, double xv) {
                         const int e = I * nv + i;
-                        x[e] = xv;
-                        const double zo = z[e], uo = u[e];
+                        const double zo = z[e], uo = u[e], qe = q[e];
                         double zn = xv + uo;
                         if (zn < 0.0) zn = 0.0;
                         z[e] = zn;
                         const double un = uo + xv - zn;
                         u[e] = un;
+                        sR[I * lv + i] = qe + rho * (zn - un) + sigma * xv;
                         const double pv = xv - zn, dv = rho * (zn - zo);
                         nr[0] += pv * pv;
                         nr[1] += dv * dv;
@@ -334,7 +368,7 @@ __global__ void __launch_bounds__(ANT) k_admm(const Inst* __restrict__ inst, con
                         nr[3] += xv * xv;
                         nr[4] += un * un;
                     });
-        block_sum_n<5, ANW>(nr, red);
+        block_sum_n<5, ANW>(nr, red);  // its first barrier also orders the R writes before GEMM 1
         ++it;
         const double ep = eps_abs + eps_rel * sqrt(fmax(nr[2], nr[3]));
         const double ed = eps_abs + eps_rel * rho * sqrt(nr[4]);
@@ -351,7 +385,7 @@ __global__ void __launch_bounds__(ANT) k_admm(const Inst* __restrict__ inst, con
     md = block_sum_w<ANW>(md, red);
     if (md > 0.0)
         for (int r = threadIdx.x; r < N; r += ANT) q0[r] = q0[r] / md;
-    if (threadIdx.x == 0) out[blockIdx.x].n_admm = it;
+    if (threadIdx.x == 0) out[order[blockIdx.x]].n_admm = it;
 }
 
 }  // namespace fpe

@@ -215,3 +215,12 @@ def test_reference_mode_up_barriers_and_down_in_put(engine, oracle, ot, B):
     K = [55.0, 60.0, 65.0]
     kw = dict(HESTON, n_s=20, n_v=16, num_insert=50, s_max=180.0, option_type=ot, barrier=B, complex_mode=1)
     _parity(K, kw, oracle, engine)
+
+
+@pytest.mark.parametrize("n_s", [120, 150])
+def test_span_mode_long_s_axis(engine, oracle, n_s):
+    # long s axes: the s-axis Jacobi eigenvectors of the initial-condition fit
+    # leave LDS (120: eigenvectors in global memory; 150: the matrix too)
+    kw = dict(HESTON, disc_mode=1, n_s=n_s, n_v=8, v_max=1.5, s_max=180.0, ic_sigma_s=0.5, ic_sigma_v=0.005,
+              option_type=8, complex_mode=1)
+    _parity([50.0, 60.0, 70.0], kw, oracle, engine)
