@@ -224,23 +224,22 @@ __global__ void __launch_bounds__(ENT) k_eig(const Inst* __restrict__ inst, cons
 
 // C = A B (N <= 64) with a K-loop, v_mfma_f64_16x16x4f64; operands typically in
 // LDS. A wave owns a 16-row strip across all (at most 4) column tiles: each
-// k-step loads one A fragment and feeds four independent accumulator chains,
-// so an operand read from memory (the s eigenvectors when they do not fit in
-// LDS) is waited for once per k-step, not once per tile. The k order matches
-// a per-tile loop, so results are identical. C(r, c, value) receives each
-// output once. NW: waves in the workgroup.
-template <int NW, class FA, class FB, class FC>
+// k-step feeds one A fragment to four independent accumulator chains.
+// AP > 0 (an A operand read from memory: the s eigenvectors when they do not
+// fit in LDS): A fragments are loaded a chunk of AP k-steps ahead of their use,
+// so the operand waits once per chunk, behind the previous chunk's MFMAs. The
+// k order is the same either way, so results are identical. C(r, c, value)
+// receives each output once. NW: waves in the workgroup.
+template <int NW, int AP = 0, class FA, class FB, class FC>
 __device__ __forceinline__ void mfma_gemm_k(int M, int N, int K, FA A, FB B, FC C) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int tm = (M + 15) >> 4, tn = (N + 15) >> 4;
     const int l16 = lane & 15, l4 = lane >> 4;
     for (int t = wave; t < tm; t += NW) {
         const int r0 = t << 4, ra = r0 + l16;
+        const bool va = ra < M;
         d4 acc[4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
-#pragma unroll 2
-        for (int k0 = 0; k0 < K; k0 += 4) {
-            const int k = k0 + l4;
-            const double a = (ra < M && k < K) ? A(ra, k) : 0.0;
+        auto kstep = [&](double a, int k) {
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 if (u < tn) {
@@ -249,6 +248,32 @@ __device__ __forceinline__ void mfma_gemm_k(int M, int N, int K, FA A, FB B, FC 
                     acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[u], 0, 0, 0);
                 }
             }
+        };
+        int k0 = 0;
+        if constexpr (AP > 0) {
+            // whole chunks: every k of the chunk is in range
+            const int kfull = (K / (4 * AP)) * (4 * AP);
+            double a[AP];
+            if (kfull > 0) {
+#pragma unroll
+                for (int j = 0; j < AP; ++j) a[j] = va ? A(ra, 4 * j + l4) : 0.0;
+            }
+#pragma unroll 1
+            for (; k0 < kfull; k0 += 4 * AP) {
+                double an[AP];
+                const bool more = k0 + 4 * AP < kfull;
+#pragma unroll
+                for (int j = 0; j < AP; ++j) an[j] = (va && more) ? A(ra, k0 + 4 * AP + 4 * j + l4) : 0.0;
+#pragma unroll
+                for (int j = 0; j < AP; ++j) kstep(a[j], k0 + 4 * j + l4);
+#pragma unroll
+                for (int j = 0; j < AP; ++j) a[j] = an[j];
+            }
+        }
+#pragma unroll 2
+        for (; k0 < K; k0 += 4) {
+            const int k = k0 + l4;
+            kstep((va && k < K) ? A(ra, k) : 0.0, k);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -331,7 +356,7 @@ __global__ void __launch_bounds__(ANT) k_admm(const Inst* __restrict__ inst, con
     const int max_iter = 50000;
     for (; it < max_iter;) {
         // T = Qs^T R
-        mfma_gemm_k<ANW>(ns, nv, ns, [&](int a, int I) { return Qs(I, a); }, [&](int I, int i) { return sR[I * lv + i]; },
+        mfma_gemm_k<ANW, 4>(ns, nv, ns, [&](int a, int I) { return Qs(I, a); }, [&](int I, int i) { return sR[I * lv + i]; },
                     [&](int a, int i, double v) { sT[a * lv + i] = v; });
         __syncthreads();
         // R = (T Qv) ./ (ls^2 lv^2 + alpha)
@@ -343,7 +368,7 @@ __global__ void __launch_bounds__(ANT) k_admm(const Inst* __restrict__ inst, con
                     });
         __syncthreads();
         // T = Qs R
-        mfma_gemm_k<ANW>(ns, nv, ns, [&](int I, int a) { return Qs(I, a); }, [&](int a, int b) { return sR[a * lv + b]; },
+        mfma_gemm_k<ANW, 4>(ns, nv, ns, [&](int I, int a) { return Qs(I, a); }, [&](int a, int b) { return sR[a * lv + b]; },
                     [&](int I, int b, double v) { sT[I * lv + b] = v; });
         __syncthreads();
         // x = T Qv^T, then the z and u updates, the five norms and the next
