@@ -25,8 +25,8 @@ namespace fpe {
 constexpr int RNT = 256;
 constexpr int RNW = RNT / 64;
 constexpr int RADAU_WG_PER_CU = 2;
-constexpr int LS = MAX_BLOCK + 1;  // LDS row stride of an m x m block
-constexpr int NRG = MAX_BLOCK / RNW;  // register columns per lane in the Gauss-Jordan inverse
+constexpr int LS = MAX_BLOCK + 1;  // LDS row stride of an m x m block (column MAX_BLOCK is padding)
+constexpr int GJ_PW = 8;           // pivot steps per Gauss-Jordan panel
 
 struct LuLds {
     // two m x m staging blocks: the pivot block and its inverse (re, im), then
@@ -34,9 +34,12 @@ struct LuLds {
     double B[2][MAX_BLOCK * LS];
     int piv[MAX_BLOCK];   // pivot row of each elimination step
     int qinv[MAX_BLOCK];  // step whose pivot row is a given row
-    double colr[2][64], coli[2][64];  // pivot column of a step (double-buffered)
-    double ipr[2], ipi[2];            // 1 / pivot
-    int pk[2];                        // pivot row
+    // Gauss-Jordan panel of GJ_PW pivot steps: column s of the panel right after
+    // step s (re, im; at [s * 64 + row]), 1 / pivot, pivot rows, rows used so far
+    double U[2][GJ_PW * 64];
+    double ip[2][GJ_PW];
+    int pv[GJ_PW];
+    unsigned long long umask;
     int singular;
     // wall-clock ticks by sub-phase (thread 0): pivot-block inverse, staging, L multiply, Schur update
     unsigned long long tk[4];
@@ -132,114 +135,263 @@ __device__ __forceinline__ T* lds_block(LuLds& L, int k) { return reinterpret_ca
 template <class T>
 __device__ __forceinline__ const T* lds_block(const LuLds& L, int k) { return reinterpret_cast<const T*>(L.B[k]); }
 
-// Gauss-Jordan inverse of the m x m block in (B[0], B[1]) by all four waves.
-// Lane i holds row i and wave w holds the columns j = w (mod 4) in registers,
-// so an elimination step is one register update per held entry plus a lane
-// broadcast of the pivot row, with one workgroup barrier per step. Partial
-// pivoting is implicit: step k takes the not-yet-used row p_k with the largest
-// |A[.][k]| (|re| + |im| in complex arithmetic; the search key drops the 6
-// lowest mantissa bits to carry the row, ties go to the lower row) and no rows
-// move. The owner wave of column k+1 updates that column and searches its
-// pivot before updating its other columns, so the search overlaps the other
-// waves' updates. Diagonal-pivot in-place Gauss-Jordan on P A leaves
-// X[p_k][j] = inv(A)[k][p_j]; the last pass unscrambles inv(A)[a][b] = X[p_a][q_b]
-// with p_{q_b} = b in place. Result in natural order in (B[0], B[1]). Sets
-// singular on a zero or non-finite pivot. Ends with a barrier.
+// Pivot search key of a row: |value| rounded to FP32 with its 7 lowest
+// significand bits replaced by a valid flag (64) and 63 - lane, so one unsigned
+// 32-bit max yields the pivot row (ties within 2^-16 relative go to the lower
+// row; non-finite magnitudes order above finite ones). Used rows: key 0.
+__device__ __forceinline__ unsigned pivot_key32(double mag, int lane) {
+    const unsigned b = __builtin_bit_cast(unsigned, float(mag));
+    return (b & ~127u) | 64u | unsigned(63 - lane);
+}
+__device__ __forceinline__ unsigned pivot_key32(float mag, int lane) {
+    const unsigned b = __builtin_bit_cast(unsigned, mag);
+    return (b & ~127u) | 64u | unsigned(63 - lane);
+}
+// Wave-uniform maximum of an unsigned key: DPP row rotations (8, 4, 2, 1) leave
+// each 16-lane row's maximum in all its lanes; the four rows combine in SGPRs.
+__device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, false));
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x124, 0xF, 0xF, false));
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x122, 0xF, 0xF, false));
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x121, 0xF, 0xF, false));
+    const unsigned a = (unsigned)__builtin_amdgcn_readlane((int)v, 0), b = (unsigned)__builtin_amdgcn_readlane((int)v, 16);
+    const unsigned c = (unsigned)__builtin_amdgcn_readlane((int)v, 32), d = (unsigned)__builtin_amdgcn_readlane((int)v, 48);
+    return max(max(a, b), max(c, d));
+}
+// 1 / d for a nonzero finite pivot: hardware reciprocal and two Newton steps
+// (within an ulp or so of the correctly rounded quotient).
+__device__ __forceinline__ double recip(double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    r = fma(r, fma(-d, r, 1.0), r);
+    return fma(r, fma(-d, r, 1.0), r);
+}
+__device__ __forceinline__ float recip(float d) {
+    float r = __builtin_amdgcn_rcpf(d);
+    return fmaf(r, fmaf(-d, r, 1.0f), r);
+}
+
+// Steps k0..k0+nb-1 of the blocked Gauss-Jordan inverse on this wave's 16
+// columns c = 16 w + lane % 16. Pivot rows follow the step-by-step recurrence
+// (every lane of a 16-lane group repeats it for its column):
+//     y_s = x_{p_s}, x_{p_s} <- y_s / piv_s, x_{p_t} <- x_{p_t} + U[p_t][s] y_s (t != s),
+// where U[.][s] is panel column s right after step s (-f / piv_s off the pivot
+// row); every other row takes x_i <- x_i + sum_s U[i][s] y_s as a rank-nb MFMA
+// update per 16-row tile, and the pivot rows are then overwritten. All LDS
+// accesses stay inside the 64 x 65 staging block: rows and columns past m are
+// padding whose contents never reach rows and columns below m.
 template <bool CPLX, class T>
-__device__ void gj_inverse(LuLds& L, int m) {
+__device__ __forceinline__ void apply_panel(LuLds& L, int m, int k0, int nb) {
+    typedef typename Mf<T>::acc A4;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int l16 = lane & 15, l4 = lane >> 4;
+    const int c = (w << 4) + l16;
+    if ((w << 4) >= m) return;  // wave-uniform: no columns here
+    T* B0 = lds_block<T>(L, 0);
+    T* B1 = lds_block<T>(L, 1);
+    const T* Ur = reinterpret_cast<const T*>(L.U[0]);
+    const T* Ui = reinterpret_cast<const T*>(L.U[1]);
+    int p[GJ_PW];
+    T zr[GJ_PW], zi[GJ_PW], yr[GJ_PW], yi[GJ_PW];
+#pragma unroll
+    for (int t = 0; t < GJ_PW; ++t) {
+        p[t] = t < nb ? L.pv[t] : 0;
+        zr[t] = B0[p[t] * LS + c];
+        zi[t] = CPLX ? B1[p[t] * LS + c] : T(0);
+        yr[t] = yi[t] = T(0);
+    }
+#pragma unroll
+    for (int s = 0; s < GJ_PW; ++s) {
+        if (s < nb) {
+            const T rr = T(L.ip[0][s]), ri = CPLX ? T(L.ip[1][s]) : T(0);
+            yr[s] = zr[s], yi[s] = zi[s];
+            zr[s] = CPLX ? yr[s] * rr - yi[s] * ri : yr[s] * rr;
+            if (CPLX) zi[s] = yr[s] * ri + yi[s] * rr;
+#pragma unroll
+            for (int t = 0; t < GJ_PW; ++t) {
+                if (t != s && t < nb) {
+                    const T ur = Ur[s * 64 + p[t]], ui = CPLX ? Ui[s * 64 + p[t]] : T(0);
+                    zr[t] += CPLX ? ur * yr[s] - ui * yi[s] : ur * yr[s];
+                    if (CPLX) zi[t] += ur * yi[s] + ui * yr[s];
+                }
+            }
+        }
+    }
+    // B operands: y_{4 kk + lane / 16} of column c
+    T br[GJ_PW / 4], bi[GJ_PW / 4];
+#pragma unroll
+    for (int kk = 0; kk < GJ_PW / 4; ++kk) {
+        br[kk] = bi[kk] = T(0);
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            if (t == l4) br[kk] = yr[4 * kk + t], bi[kk] = yi[4 * kk + t];
+    }
+    const int tm = (m + 15) >> 4;
+    for (int tt = 0; tt < tm; ++tt) {
+        const int ra = (tt << 4) + l16;
+        A4 accr, acci;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int r = (tt << 4) + Mf<T>::row(lane, j);
+            accr[j] = B0[r * LS + c];
+            acci[j] = CPLX ? B1[r * LS + c] : T(0);
+        }
+#pragma unroll
+        for (int kk = 0; kk < GJ_PW / 4; ++kk) {
+            if (4 * kk < nb) {
+                const T ar = Ur[(4 * kk + l4) * 64 + ra], ai = CPLX ? Ui[(4 * kk + l4) * 64 + ra] : T(0);
+                mfma_c(accr, acci, ar, ai, br[kk], bi[kk], CPLX);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int r = (tt << 4) + Mf<T>::row(lane, j);
+            B0[r * LS + c] = accr[j];
+            if (CPLX) B1[r * LS + c] = acci[j];
+        }
+    }
+    // pivot rows last (lane group l4 writes t = l4, l4 + 4, ...; t >= nb to padding)
+#pragma unroll
+    for (int h = 0; h < GJ_PW / 4; ++h) {
+        T vr = T(0), vi = T(0);
+        int pr = 0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            if (t == l4) vr = zr[4 * h + t], vi = zi[4 * h + t], pr = p[4 * h + t];
+        const int o = (4 * h + l4 < nb) ? pr * LS + c : MAX_BLOCK;  // row 0, padding column
+        B0[o] = vr;
+        if (CPLX) B1[o] = vi;
+    }
+}
+
+// Gauss-Jordan inverse of the m x m block in (B[0], B[1]), blocked by panels
+// of GJ_PW pivot steps. Partial pivoting is implicit: step k takes the
+// not-yet-used row p_k with the largest |A[.][k]| (|re| + |im| in complex
+// arithmetic, compared in FP32 to 2^-16 relative, ties to the lower row) and no
+// rows move; diagonal-pivot in-place Gauss-Jordan on P A leaves
+// X[p_k][j] = inv(A)[k][p_j], which the last pass unscrambles into natural
+// order with p_{q_b} = b.
+// Per panel (columns k0..k0+GJ_PW-1):
+//  1. the wave owning the panel's 16-column tile runs the steps on the panel
+//     columns alone, in registers (lane = row), and publishes each column right
+//     after its step, 1 / pivot and the pivot rows;
+//  2. every wave applies the steps to its own 16 columns (apply_panel);
+//  3. the panel wave stores the panel columns last.
+// A wave only reads and writes its own columns in 2-3, so two barriers per
+// panel order everything. Sets singular on a zero or non-finite pivot. Ends
+// with a barrier.
+template <bool CPLX, class T>
+__device__ __forceinline__ void gj_inverse(LuLds& L, int m) {
+    m = uniform(m);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const bool lrow = lane < m;
     T* B0 = lds_block<T>(L, 0);
     T* B1 = lds_block<T>(L, 1);
-    T xr[NRG], xi[NRG];
-#pragma unroll
-    for (int r = 0; r < NRG; ++r) {
-        const int j = RNW * r + w;
-        xr[r] = (lrow && j < m) ? B0[lane * LS + j] : T(0);
-        xi[r] = (CPLX && lrow && j < m) ? B1[lane * LS + j] : T(0);
-    }
-    bool used = !lrow;
+    T* Ur = reinterpret_cast<T*>(L.U[0]);
+    T* Ui = reinterpret_cast<T*>(L.U[1]);
     if (threadIdx.x < MAX_BLOCK) L.qinv[threadIdx.x] = 0;  // stays in range if a bad pivot search repeats a row
-
-    // pivot search of column k (owner wave): publishes the column, the pivot row and 1/pivot in buffer b
-    auto search = [&](int k, T cr, T ci, int b) {
-        const T mag = CPLX ? fabs(cr) + fabs(ci) : fabs(cr);
-        T key = used ? T(-1) : pivot_key(mag, lane);
-        key = wave_max_dpp(key);
-        const int p = __builtin_amdgcn_readfirstlane(key_lane(key));
-        const T dr = readlane_d(cr, p), di = CPLX ? readlane_d(ci, p) : T(0);
-        T ir, ii;
-        if (CPLX) {
-            const T den = dr * dr + di * di;
-            ir = den > T(0) ? dr / den : T(0);
-            ii = den > T(0) ? -di / den : T(0);
-        } else {
-            ir = dr != T(0) ? T(1) / dr : T(0);
-            ii = T(0);
-        }
-        L.colr[b][lane] = cr;
-        if (CPLX) L.coli[b][lane] = ci;
-        if (lane == 0) {
-            L.pk[b] = p;
-            L.ipr[b] = ir;
-            L.ipi[b] = ii;
-            L.piv[k] = p;
-            const T mp = CPLX ? fabs(dr) + fabs(di) : fabs(dr);
-            if (!(key > T(0)) || !(mp > T(0)) || !(mp < T(INFINITY))) L.singular = 1;
-        }
-    };
-    if (w == 0) search(0, xr[0], xi[0], 0);
-    for (int k = 0; k < m; ++k) {
-        const int b = k & 1;
+    if (threadIdx.x == 0) L.umask = 0;
+    for (int k0 = 0; k0 < m; k0 += GJ_PW) {
+        const int nb = m - k0 < GJ_PW ? m - k0 : GJ_PW;
+        const int wp = k0 >> 4;
+        T cr[GJ_PW], ci[GJ_PW];  // panel columns (panel wave, lane = row)
         __syncthreads();
-        const int p = __builtin_amdgcn_readfirstlane(L.pk[b]);
-        const T fr = T(L.colr[b][lane]), fi = CPLX ? T(L.coli[b][lane]) : T(0);
-        const T ir = T(L.ipr[b]), ii = CPLX ? T(L.ipi[b]) : T(0);
-        const bool isp = lane == p;
-        if (isp) used = true;
-        // in-place step on the entry (row lane, column j): pivot row -> row / pivot,
-        // column k -> -f / pivot (1 / pivot on the pivot row), rest -> x - f * row_p / pivot
-        auto upd = [&](T& vr, T& vi, int j) {
-            T sr, si;
-            if (j == k) {
-                sr = ir, si = ii;
-            } else {
-                const T pr = readlane_d(vr, p);
-                const T pi_ = CPLX ? readlane_d(vi, p) : T(0);
-                sr = CPLX ? pr * ir - pi_ * ii : pr * ir;
-                si = CPLX ? pr * ii + pi_ * ir : T(0);
+        if (w == wp) {
+            T ir[GJ_PW], ii[GJ_PW];
+            int pv[GJ_PW];
+#pragma unroll
+            for (int s = 0; s < GJ_PW; ++s) {
+                const int col = s < nb ? k0 + s : MAX_BLOCK;  // padding column past the block
+                cr[s] = lrow ? B0[lane * LS + col] : T(0);
+                ci[s] = (CPLX && lrow) ? B1[lane * LS + col] : T(0);
+                ir[s] = ii[s] = T(0);
+                pv[s] = 0;
             }
-            if (isp) {
-                vr = sr;
-                if (CPLX) vi = si;
-            } else {
-                const T br = (j == k) ? T(0) : vr, bi = (j == k) ? T(0) : vi;
-                vr = br - (CPLX ? fr * sr - fi * si : fr * sr);
-                if (CPLX) vi = bi - (fr * si + fi * sr);
+            const unsigned long long um = L.umask;
+            bool used = !lrow || ((um >> lane) & 1ull);
+            bool bad = false;
+#pragma unroll
+            for (int s = 0; s < GJ_PW; ++s) {
+                if (s < nb) {
+                    const T mag = CPLX ? fabs(cr[s]) + fabs(ci[s]) : fabs(cr[s]);
+                    const unsigned key = wave_max_u32(used ? 0u : pivot_key32(mag, lane));
+                    const int p = 63 - int(key & 63u);
+                    const T dr = readlane_d(cr[s], p), di = CPLX ? readlane_d(ci[s], p) : T(0);
+                    const T mp = CPLX ? fabs(dr) + fabs(di) : fabs(dr);
+                    const bool ok = key != 0u && mp > T(0) && mp < T(INFINITY);
+                    bad = bad || !ok;
+                    T rr, ri;
+                    if (CPLX) {
+                        const T rd = recip(dr * dr + di * di);
+                        rr = ok ? dr * rd : T(0);
+                        ri = ok ? -di * rd : T(0);
+                    } else {
+                        rr = ok ? recip(dr) : T(0);
+                        ri = T(0);
+                    }
+                    const bool isp = lane == p;
+                    if (isp) used = true;
+                    const T gr = cr[s], gi = ci[s];
+                    ir[s] = rr, ii[s] = ri, pv[s] = p;
+#pragma unroll
+                    for (int t = s + 1; t < GJ_PW; ++t) {
+                        if (t < nb) {
+                            const T pr = readlane_d(cr[t], p);
+                            const T pi_ = CPLX ? readlane_d(ci[t], p) : T(0);
+                            const T sr = CPLX ? pr * rr - pi_ * ri : pr * rr;
+                            const T si = CPLX ? pr * ri + pi_ * rr : T(0);
+                            const T nr = cr[t] - (CPLX ? gr * sr - gi * si : gr * sr);
+                            const T ni = CPLX ? ci[t] - (gr * si + gi * sr) : T(0);
+                            cr[t] = isp ? sr : nr;
+                            if (CPLX) ci[t] = isp ? si : ni;
+                        }
+                    }
+#pragma unroll
+                    for (int t = 0; t < s; ++t) {
+                        const T pr = readlane_d(cr[t], p);
+                        const T pi_ = CPLX ? readlane_d(ci[t], p) : T(0);
+                        const T sr = CPLX ? pr * rr - pi_ * ri : pr * rr;
+                        const T si = CPLX ? pr * ri + pi_ * rr : T(0);
+                        const T nr = cr[t] - (CPLX ? gr * sr - gi * si : gr * sr);
+                        const T ni = CPLX ? ci[t] - (gr * si + gi * sr) : T(0);
+                        cr[t] = isp ? sr : nr;
+                        if (CPLX) ci[t] = isp ? si : ni;
+                    }
+                    const T nr = -(CPLX ? gr * rr - gi * ri : gr * rr);
+                    const T ni = CPLX ? -(gr * ri + gi * rr) : T(0);
+                    cr[s] = isp ? rr : nr;
+                    if (CPLX) ci[s] = isp ? ri : ni;
+                    Ur[s * 64 + lane] = cr[s];
+                    if (CPLX) Ui[s * 64 + lane] = ci[s];
+                } else {
+                    Ur[s * 64 + lane] = T(0);
+                    if (CPLX) Ui[s * 64 + lane] = T(0);
+                }
             }
-        };
-        const int k1 = k + 1;
-        if (k1 < m && w == (k1 & (RNW - 1))) {
-            const int r1 = k1 / RNW;
-            T cr = T(0), ci = T(0);
+            if (lane == 0) {
+                unsigned long long nm = um;
 #pragma unroll
-            for (int r = 0; r < NRG; ++r)
-                if (r == r1) cr = xr[r], ci = xi[r];
-            upd(cr, ci, k1);
-            search(k1, cr, ci, b ^ 1);
+                for (int s = 0; s < GJ_PW; ++s) {
+                    if (s < nb) {
+                        L.ip[0][s] = ir[s], L.ip[1][s] = ii[s];
+                        L.pv[s] = pv[s];
+                        L.piv[k0 + s] = pv[s];
+                        nm |= 1ull << pv[s];
+                    }
+                }
+                L.umask = nm;
+                if (bad) L.singular = 1;
+            }
         }
+        __syncthreads();
+        apply_panel<CPLX, T>(L, m, k0, nb);
+        if (w == wp) {
 #pragma unroll
-        for (int r = 0; r < NRG; ++r) {
-            const int j = RNW * r + w;
-            if (j < m) upd(xr[r], xi[r], j);
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < NRG; ++r) {
-        const int j = RNW * r + w;
-        if (lrow && j < m) {
-            B0[lane * LS + j] = xr[r];
-            if (CPLX) B1[lane * LS + j] = xi[r];
+            for (int s = 0; s < GJ_PW; ++s) {
+                if (s < nb) {
+                    B0[lane * LS + k0 + s] = cr[s];  // rows past m: padding
+                    if (CPLX) B1[lane * LS + k0 + s] = ci[s];
+                }
+            }
         }
     }
     __syncthreads();
@@ -283,7 +435,7 @@ __device__ __forceinline__ long long stacked_row(long long ps, int ld, int I, in
 // panels (a wave takes 16-row strips; their fragments stay in registers), the
 // inverse from LDS (B[0], B[1]).
 template <bool CPLX, class T>
-__device__ void lower_multiply(T* __restrict__ Pr, T* __restrict__ Pi, long long ps, int ld, int I, int m, int nU,
+__device__ void lower_multiply(GAS T* __restrict__ Pr, GAS T* __restrict__ Pi, long long ps, int ld, int I, int m, int nU,
                                const LuLds& L) {
     typedef typename Mf<T>::acc A4;
     constexpr int TG = CPLX ? 2 : 4;  // column tiles per accumulator group
@@ -358,7 +510,7 @@ __device__ void lower_multiply(T* __restrict__ Pr, T* __restrict__ Pi, long long
 // tiles of a K block run in groups (C tiles loaded up front, one independent
 // accumulator chain per tile).
 template <bool CPLX, class T>
-__device__ void schur_pass(T* __restrict__ Pr, T* __restrict__ Pi, long long ps, int ld, int I, int m, int nU,
+__device__ void schur_pass(GAS T* __restrict__ Pr, GAS T* __restrict__ Pi, long long ps, int ld, int I, int m, int nU,
                            int kb0, int nkb, const LuLds& L) {
     typedef typename Mf<T>::acc A4;
     constexpr int TG = CPLX ? 2 : 4;  // column tiles per accumulator group
@@ -494,7 +646,10 @@ __device__ void assemble_E(const double* __restrict__ Ms, const double* __restri
 
 // Factor the panels in place. Returns nonzero if a diagonal block was singular.
 template <bool CPLX, class T>
-__device__ __noinline__ int block_lu(T* __restrict__ Pr, T* __restrict__ Pi, int ns, int m, LuLds& L) {
+__device__ __noinline__ int block_lu(T* __restrict__ Pr_, T* __restrict__ Pi_, int ns, int m, LuLds& L) {
+    GAS T* const Pr = gas(Pr_);
+    GAS T* const Pi = gas(Pi_);
+    ns = uniform(ns), m = uniform(m);
     const int ld = 7 * m;
     const long long ps = (long long)m * ld;
     const int tid = threadIdx.x;
@@ -512,8 +667,8 @@ __device__ __noinline__ int block_lu(T* __restrict__ Pr, T* __restrict__ Pi, int
     T* B0 = lds_block<T>(L, 0);
     T* B1 = lds_block<T>(L, 1);
     for (int I = 0; I < ns; ++I) {
-        T* PIr = Pr + I * ps;
-        T* PIi = Pi + I * ps;
+        GAS T* PIr = Pr + I * ps;
+        GAS T* PIi = Pi + I * ps;
         for (int e = tid; e < m * m; e += RNT) {
             const int i = e / m, j = e - i * m;
             B0[i * LS + j] = PIr[(long long)i * ld + 3 * m + j];
@@ -565,8 +720,13 @@ __device__ __noinline__ int block_lu(T* __restrict__ Pr, T* __restrict__ Pi, int
 // at least 10 * 64 doubles. Factors of type T (FP64, or FP32 for the
 // mixed-precision path) are widened to FP64; all arithmetic is FP64.
 template <bool CPLX, class T>
-__device__ __noinline__ void block_solve1(const T* __restrict__ Pr, const T* __restrict__ Pi, int ns, int m,
-                                          double* __restrict__ xr, double* __restrict__ xi, double* ring) {
+__device__ __noinline__ void block_solve1(const T* __restrict__ Pr_, const T* __restrict__ Pi_, int ns, int m,
+                                          double* __restrict__ xr_, double* __restrict__ xi_, double* ring) {
+    GAS const T* const Pr = gas(Pr_);
+    GAS const T* const Pi = gas(Pi_);
+    GAS double* const xr = gas(xr_);
+    GAS double* const xi = gas(xi_);
+    ns = uniform(ns), m = uniform(m);
     const int ld = 7 * m;
     const long long ps = (long long)m * ld;
     constexpr int TPR = RNT / MAX_BLOCK;  // threads per row
@@ -578,8 +738,8 @@ __device__ __noinline__ void block_solve1(const T* __restrict__ Pr, const T* __r
     double* tr = ring + 8 * 64;
     double* ti = ring + 9 * 64;
     double a0[U], b0[U], a1[U], b1[U];
-    const T* pr = nullptr;
-    const T* pi = nullptr;
+    GAS const T* pr = nullptr;
+    GAS const T* pi = nullptr;
     // factor entries (row i, columns col + q + 4u) of the current block row
     auto load = [&](int col, bool ok, double* a, double* b) {
 #pragma unroll
@@ -613,6 +773,12 @@ __device__ __noinline__ void block_solve1(const T* __restrict__ Pr, const T* __r
             if (CPLX) ai += __shfl_xor(ai, o, 64);
         }
     };
+    // The first block rows contract zero factor entries with ring slots of blocks
+    // left of the matrix, which this solve never writes: start from zeros (the
+    // staging LDS the ring reuses holds arbitrary values).
+    __syncthreads();
+    for (int e = tid; e < 8 * 64; e += RNT) ring[e] = 0.0;
+    __syncthreads();
     // forward: y_I = b_I - sum_{d=0..2} L_{I, I-3+d} y_{I-3+d} (ring slot of block J: J & 3)
     for (int I = 0; I < ns; ++I) {
         pr = Pr + I * ps + (long long)i * ld;

@@ -16,6 +16,16 @@ constexpr int NT = 256;        // threads per workgroup
 constexpr int NWAVE = NT / 64;
 constexpr int MAX_BLOCK = 64;  // largest n_vb handled by the block-banded LU
 
+// Global-memory pointer type. Functions that are not inlined into the kernel
+// receive generic pointers, which compile to FLAT accesses (counted against the
+// LDS wait counter too); casting to this type gives global_load/store.
+#define GAS __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ GAS T* gas(T* p) { return (GAS T*)p; }
+// A wave-uniform int argument of a non-inlined function, made visibly uniform
+// so that conditions on it compile to scalar branches.
+__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 // Per-instance descriptor (one Fokker-Planck solve). Offsets are element
