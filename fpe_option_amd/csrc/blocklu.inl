@@ -644,11 +644,45 @@ __device__ void assemble_E(const double* __restrict__ Ms, const double* __restri
     }
 }
 
+// Copy the m x m block at (sr, si) (row stride ld) into the LDS staging block
+// (dr, di) (row stride LS): wave w takes rows w, w + RNW, ... with lane = column,
+// so each load instruction reads one contiguous row segment, and the loads of
+// half the rows are issued before their stores (two memory latencies per
+// block). Padding columns of the staged rows are written with zeros.
+template <bool CPLX, class T>
+__device__ __forceinline__ void stage_in(T* dr, T* di, GAS const T* sr, GAS const T* si, int ld, int m) {
+    constexpr int R = MAX_BLOCK / RNW / 2;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        T vr[R], vi[R];
+#pragma unroll
+        for (int u = 0; u < R; ++u) vr[u] = vi[u] = T(0);
+        if (lane < m) {
+#pragma unroll
+            for (int u = 0; u < R; ++u) {
+                const int r = w + RNW * (u + R * h);
+                if (r < m) {
+                    vr[u] = sr[r * ld + lane];
+                    if (CPLX) vi[u] = si[r * ld + lane];
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            const int r = w + RNW * (u + R * h);
+            if (r < m) {
+                dr[r * LS + lane] = vr[u];
+                if (CPLX) di[r * LS + lane] = vi[u];
+            }
+        }
+    }
+}
 // Factor the panels in place. Returns nonzero if a diagonal block was singular.
 template <bool CPLX, class T>
 __device__ __noinline__ int block_lu(T* __restrict__ Pr_, T* __restrict__ Pi_, int ns, int m, LuLds& L) {
-    GAS T* const Pr = gas(Pr_);
-    GAS T* const Pi = gas(Pi_);
+    GAS T* const Pr = uniform(gas(Pr_));
+    GAS T* const Pi = uniform(gas(Pi_));
     ns = uniform(ns), m = uniform(m);
     const int ld = 7 * m;
     const long long ps = (long long)m * ld;
@@ -694,11 +728,7 @@ __device__ __noinline__ int block_lu(T* __restrict__ Pr_, T* __restrict__ Pi_, i
                 T* Br = lds_block<T>(L, CPLX ? 2 * kb : kb);
                 T* Bi = CPLX ? lds_block<T>(L, 2 * kb + 1) : nullptr;
                 const int col = (4 + kb0 + kb) * m;
-                for (int e = tid; e < m * m; e += RNT) {
-                    const int i = e / m, j = e - i * m;
-                    Br[i * LS + j] = PIr[(long long)i * ld + col + j];
-                    if (CPLX) Bi[i * LS + j] = PIi[(long long)i * ld + col + j];
-                }
+                stage_in<CPLX, T>(Br, Bi, PIr + col, CPLX ? PIi + col : nullptr, ld, m);
             }
             __syncthreads();
             lap(1);
@@ -722,10 +752,10 @@ __device__ __noinline__ int block_lu(T* __restrict__ Pr_, T* __restrict__ Pi_, i
 template <bool CPLX, class T>
 __device__ __noinline__ void block_solve1(const T* __restrict__ Pr_, const T* __restrict__ Pi_, int ns, int m,
                                           double* __restrict__ xr_, double* __restrict__ xi_, double* ring) {
-    GAS const T* const Pr = gas(Pr_);
-    GAS const T* const Pi = gas(Pi_);
-    GAS double* const xr = gas(xr_);
-    GAS double* const xi = gas(xi_);
+    GAS const T* const Pr = uniform(gas(Pr_));
+    GAS const T* const Pi = uniform(gas(Pi_));
+    GAS double* const xr = uniform(gas(xr_));
+    GAS double* const xi = uniform(gas(xi_));
     ns = uniform(ns), m = uniform(m);
     const int ld = 7 * m;
     const long long ps = (long long)m * ld;

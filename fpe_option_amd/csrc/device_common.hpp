@@ -25,6 +25,15 @@ __device__ __forceinline__ GAS T* gas(T* p) { return (GAS T*)p; }
 // A wave-uniform int argument of a non-inlined function, made visibly uniform
 // so that conditions on it compile to scalar branches.
 __device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+// A wave-uniform global pointer argument, moved to scalar registers: loads then
+// take a scalar base and a 32-bit vector offset.
+template <class T>
+__device__ __forceinline__ GAS T* uniform(GAS T* p) {
+    const unsigned long long v = (unsigned long long)p;
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
+    return (GAS T*)(((unsigned long long)hi << 32) | lo);
+}
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
