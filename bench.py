@@ -120,6 +120,25 @@ def surrogate_leg(plain_value, n=1_000_000, audit_frac=0.01, n_out=256, seed=202
                         f"{audit_frac:.0%} audit"}
 
 
+def mixed_leg(api, cols, prices, status, plain_value, n, precision, path):
+    """The same batch with FP32 factors and FP64-refined stage solves
+    (precision 1), one timed step, prices compared with the plain FP64 step's."""
+    mcols = dict(cols, precision=precision, complex_mode=1)
+    t = time.perf_counter()
+    mprices, mstatus, mst = api.price_batch(return_stats=True, **mcols)
+    mwall = time.perf_counter() - t
+    ok = (mstatus == 0) & (status == 0)
+    d = np.abs(mprices - prices)[ok]
+    return {"value": round(n / mwall, 3), "unit": "options/s", "ms_per_step": round(1e3 * mwall, 1),
+            "vs_plain_fp64": round((n / mwall) / plain_value, 3), "precision": precision, "path": path,
+            "max_abs_dprice_vs_fp64": float(np.max(d)), "p99_abs_dprice_vs_fp64": float(np.percentile(d, 99)),
+            "refine_rounds_per_stage_solve": round(mst["n_refine"] / max(1, 2 * mst["n_newton"] +
+                                                                          mst["n_err_solves"]), 3),
+            "newton_iterations": int(mst["n_newton"]), "lu_refreshes": int(mst["n_lu"]),
+            "phases_ms": {k: round(mst[k], 1) for k in ("ms_ic", "ms_radau")},
+            "status_ok": int(np.sum(mstatus == 0))}
+
+
 def copy_calibration(mbytes):
     """One plain device-to-device copy of a known size, for scaling the
     FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
@@ -207,21 +226,9 @@ def main():
         "cpu_baseline": None,
     }
     if not args.no_mixed:
-        mcols = dict(cols, precision=1, complex_mode=1)
-        t = time.perf_counter()
-        mprices, mstatus, mst = api.price_batch(return_stats=True, **mcols)
-        mwall = time.perf_counter() - t
-        ok = (mstatus == 0) & (status == 0)
-        d = np.abs(mprices - prices)[ok]
-        line["mixed"] = {"value": round(args.n / mwall, 3), "unit": "options/s", "ms_per_step": round(1e3 * mwall, 1),
-                         "vs_plain_fp64": round((args.n / mwall) / value, 3),
-                         "path": "FP32 block factorisation on v_mfma_f32_16x16x4f32, FP64 refinement of every "
-                                 "stage solve to |r| <= 1e-12 |b|; FP64 Newton residuals, error control, payoff",
-                         "max_abs_dprice_vs_fp64": float(np.max(d)), "p99_abs_dprice_vs_fp64": float(np.percentile(d, 99)),
-                         "refine_rounds_per_stage_solve": round(mst["n_refine"] / max(1, 2 * mst["n_newton"] +
-                                                                                       mst["n_err_solves"]), 3),
-                         "phases_ms": {k: round(mst[k], 1) for k in ("ms_ic", "ms_radau")},
-                         "status_ok": int(np.sum(mstatus == 0))}
+        line["mixed"] = mixed_leg(api, cols, prices, status, value, args.n, 1,
+                                  "FP32 block factorisation on v_mfma_f32_16x16x4f32, FP64 refinement of every "
+                                  "stage solve to |r| <= 1e-12 |b|; FP64 Newton residuals, error control, payoff")
     if not args.no_surrogate:
         line["mixed_surrogate"] = surrogate_leg(value)
     if cpu is not None:

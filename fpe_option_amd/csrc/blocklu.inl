@@ -27,6 +27,8 @@ constexpr int RNW = RNT / 64;
 constexpr int RADAU_WG_PER_CU = 2;
 constexpr int LS = MAX_BLOCK + 1;  // LDS row stride of an m x m block (column MAX_BLOCK is padding)
 constexpr int GJ_PW = 8;           // pivot steps per Gauss-Jordan panel
+constexpr int GJ_US = 65;          // LDS stride of a panel column (odd: the MFMA operand reads of
+                                   // four lane groups fall in different banks)
 
 struct LuLds {
     // two m x m staging blocks: the pivot block and its inverse (re, im), then
@@ -35,8 +37,8 @@ struct LuLds {
     int piv[MAX_BLOCK];   // pivot row of each elimination step
     int qinv[MAX_BLOCK];  // step whose pivot row is a given row
     // Gauss-Jordan panel of GJ_PW pivot steps: column s of the panel right after
-    // step s (re, im; at [s * 64 + row]), 1 / pivot, pivot rows, rows used so far
-    double U[2][GJ_PW * 64];
+    // step s (re, im; at [s * GJ_US + row]), 1 / pivot, pivot rows, rows used so far
+    double U[2][GJ_PW * GJ_US];
     double ip[2][GJ_PW];
     int pv[GJ_PW];
     unsigned long long umask;
@@ -209,7 +211,7 @@ __device__ __forceinline__ void apply_panel(LuLds& L, int m, int k0, int nb) {
 #pragma unroll
             for (int t = 0; t < GJ_PW; ++t) {
                 if (t != s && t < nb) {
-                    const T ur = Ur[s * 64 + p[t]], ui = CPLX ? Ui[s * 64 + p[t]] : T(0);
+                    const T ur = Ur[s * GJ_US + p[t]], ui = CPLX ? Ui[s * GJ_US + p[t]] : T(0);
                     zr[t] += CPLX ? ur * yr[s] - ui * yi[s] : ur * yr[s];
                     if (CPLX) zi[t] += ur * yi[s] + ui * yr[s];
                 }
@@ -238,7 +240,7 @@ __device__ __forceinline__ void apply_panel(LuLds& L, int m, int k0, int nb) {
 #pragma unroll
         for (int kk = 0; kk < GJ_PW / 4; ++kk) {
             if (4 * kk < nb) {
-                const T ar = Ur[(4 * kk + l4) * 64 + ra], ai = CPLX ? Ui[(4 * kk + l4) * 64 + ra] : T(0);
+                const T ar = Ur[(4 * kk + l4) * GJ_US + ra], ai = CPLX ? Ui[(4 * kk + l4) * GJ_US + ra] : T(0);
                 mfma_c(accr, acci, ar, ai, br[kk], bi[kk], CPLX);
             }
         }
@@ -360,11 +362,11 @@ __device__ __forceinline__ void gj_inverse(LuLds& L, int m) {
                     const T ni = CPLX ? -(gr * ri + gi * rr) : T(0);
                     cr[s] = isp ? rr : nr;
                     if (CPLX) ci[s] = isp ? ri : ni;
-                    Ur[s * 64 + lane] = cr[s];
-                    if (CPLX) Ui[s * 64 + lane] = ci[s];
+                    Ur[s * GJ_US + lane] = cr[s];
+                    if (CPLX) Ui[s * GJ_US + lane] = ci[s];
                 } else {
-                    Ur[s * 64 + lane] = T(0);
-                    if (CPLX) Ui[s * 64 + lane] = T(0);
+                    Ur[s * GJ_US + lane] = T(0);
+                    if (CPLX) Ui[s * GJ_US + lane] = T(0);
                 }
             }
             if (lane == 0) {
@@ -465,6 +467,35 @@ __device__ void lower_multiply(GAS T* __restrict__ Pr, GAS T* __restrict__ Pi, l
             vc[j] = r < M;
             int dJ;
             oc[j] = vc[j] ? stacked_row(ps, ld, I, m, r, dJ) + (3 - dJ) * m : 0;
+        }
+        if constexpr (CPLX) {
+            // complex products with three real ones per k-step (re = p1 - p2,
+            // im = p3 - p1 - p2): three independent chains per column tile
+            for (int u = 0; u < tn; ++u) {
+                const int c = (u << 4) + l16;
+                A4 p1 = A4{0, 0, 0, 0}, p2 = A4{0, 0, 0, 0}, p3 = A4{0, 0, 0, 0};
+#pragma unroll
+                for (int s = 0; s < MAX_BLOCK / 4; ++s) {
+                    if (s < tk) {
+                        const int k = 4 * s + l4;
+                        const bool ok = c < m && k < m;
+                        const T br = ok ? B0[k * LS + c] : T(0);
+                        const T bi = ok ? B1[k * LS + c] : T(0);
+                        p1 = Mf<T>::mfma(ar[s], br, p1);
+                        p2 = Mf<T>::mfma(ai[s], bi, p2);
+                        p3 = Mf<T>::mfma(ar[s] + ai[s], br + bi, p3);
+                    }
+                }
+                if (c < m) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        if (vc[j]) {
+                            Pr[oc[j] + c] = p1[j] - p2[j];
+                            Pi[oc[j] + c] = p3[j] - p1[j] - p2[j];
+                        }
+                }
+            }
+            continue;
         }
         // TG column tiles of the strip at a time: independent accumulator chains
         for (int ug = 0; ug < tn; ug += TG) {

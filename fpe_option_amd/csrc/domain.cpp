@@ -189,6 +189,8 @@ bool option_is_valid(const OptionSpec& o, std::string* why) {
         return fail("vanilla options take barrier = 0");
     }
     if (o.n_s < 4 || o.n_s > 256 || o.n_v < 4 || o.n_v > 256) return fail("n_s, n_v must be in [4, 256]");
+    if (o.complex_mode < 0 || o.complex_mode > 1) return fail("complex_mode must be 0 (GMRES) or 1 (direct)");
+    if (o.precision < 0 || o.precision > 1) return fail("precision must be 0 (FP64) or 1 (mixed)");
     return true;
 }
 

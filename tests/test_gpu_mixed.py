@@ -52,3 +52,15 @@ def test_mixed_in_option_parity(engine):
     doc, _ = _run(K, option_type=2, barrier=50.0, **base)
     dic, _ = _run(K, option_type=0, barrier=50.0, **base)
     np.testing.assert_allclose(dic + doc, van, rtol=0, atol=1e-13)
+
+
+def test_unknown_precision_and_complex_mode_are_rejected(engine):
+    from fpe_option_amd import api
+    prices, status = api.price_batch(precision=np.array([0, 2, 1, -1], dtype=np.int32), complex_mode=1,
+                                     strike=60.0, **HESTON, n_s=16, n_v=16, num_insert=50, s_max=180.0,
+                                     option_type=8)
+    assert status[0] == 0 and status[2] == 0
+    assert status[1] != 0 and status[3] != 0
+    _, status = api.price_batch(complex_mode=np.array([1, 2], dtype=np.int32), strike=60.0, **HESTON, n_s=16,
+                                n_v=16, num_insert=50, s_max=180.0, option_type=8)
+    assert status[0] == 0 and status[1] != 0
