@@ -709,6 +709,47 @@ __device__ __forceinline__ void stage_in(T* dr, T* di, GAS const T* sr, GAS cons
         }
     }
 }
+// The reverse copy: LDS block (dr, di) to (sr, si), half the rows at a time.
+template <bool CPLX, class T>
+__device__ __forceinline__ void stage_out(const T* dr, const T* di, GAS T* sr, GAS T* si, int ld, int m) {
+    constexpr int R = MAX_BLOCK / RNW / 2;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        T vr[R], vi[R];
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            const int r = w + RNW * (u + R * h);
+            vr[u] = r < m ? dr[r * LS + lane] : T(0);
+            vi[u] = (CPLX && r < m) ? di[r * LS + lane] : T(0);
+        }
+        if (lane < m) {
+#pragma unroll
+            for (int u = 0; u < R; ++u) {
+                const int r = w + RNW * (u + R * h);
+                if (r < m) {
+                    sr[r * ld + lane] = vr[u];
+                    if (CPLX) si[r * ld + lane] = vi[u];
+                }
+            }
+        }
+    }
+}
+// The pivot block's copies in and out of LDS as calls of their own, so that
+// their registers do not add to the factorisation loop's.
+template <bool CPLX, class T>
+__device__ __noinline__ void pivot_block_in(LuLds& L, const T* sr, const T* si, int ld, int m) {
+    m = uniform(m), ld = uniform(ld);
+    stage_in<CPLX, T>(lds_block<T>(L, 0), lds_block<T>(L, 1), uniform(gas(sr)), CPLX ? uniform(gas(si)) : nullptr,
+                      ld, m);
+}
+template <bool CPLX, class T>
+__device__ __noinline__ void pivot_block_out(LuLds& L, T* sr, T* si, int ld, int m) {
+    m = uniform(m), ld = uniform(ld);
+    stage_out<CPLX, T>(lds_block<T>(L, 0), lds_block<T>(L, 1), uniform(gas(sr)), CPLX ? uniform(gas(si)) : nullptr,
+                       ld, m);
+}
+
 // Factor the panels in place. Returns nonzero if a diagonal block was singular.
 template <bool CPLX, class T>
 __device__ __noinline__ int block_lu(T* __restrict__ Pr_, T* __restrict__ Pi_, int ns, int m, LuLds& L) {
@@ -734,20 +775,12 @@ __device__ __noinline__ int block_lu(T* __restrict__ Pr_, T* __restrict__ Pi_, i
     for (int I = 0; I < ns; ++I) {
         GAS T* PIr = Pr + I * ps;
         GAS T* PIi = Pi + I * ps;
-        for (int e = tid; e < m * m; e += RNT) {
-            const int i = e / m, j = e - i * m;
-            B0[i * LS + j] = PIr[(long long)i * ld + 3 * m + j];
-            if (CPLX) B1[i * LS + j] = PIi[(long long)i * ld + 3 * m + j];
-        }
+        pivot_block_in<CPLX, T>(L, (const T*)(PIr + 3 * m), CPLX ? (const T*)(PIi + 3 * m) : nullptr, ld, m);
         __syncthreads();
         lap(1);
         gj_inverse<CPLX, T>(L, m);
         lap(0);
-        for (int e = tid; e < m * m; e += RNT) {
-            const int i = e / m, j = e - i * m;
-            PIr[(long long)i * ld + 3 * m + j] = B0[i * LS + j];
-            if (CPLX) PIi[(long long)i * ld + 3 * m + j] = B1[i * LS + j];
-        }
+        pivot_block_out<CPLX, T>(L, (T*)(PIr + 3 * m), CPLX ? (T*)(PIi + 3 * m) : nullptr, ld, m);
         const int nU = (ns - 1 - I) < 3 ? (ns - 1 - I) : 3;
         if (nU == 0) break;
         lower_multiply<CPLX, T>(Pr, Pi, ps, ld, I, m, nU, L);
