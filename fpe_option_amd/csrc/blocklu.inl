@@ -92,45 +92,6 @@ __device__ __forceinline__ float readlane_d(float x, int l) {
     return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), l));
 }
 
-template <int CTRL>
-__device__ __forceinline__ float dpp_d(float x) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, 0xF, 0xF, false));
-}
-
-template <int CTRL>
-__device__ __forceinline__ double dpp_d(double x) {
-    const unsigned long long v = __builtin_bit_cast(unsigned long long, x);
-    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, int(unsigned(v)), CTRL, 0xF, 0xF, false);
-    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, int(unsigned(v >> 32)), CTRL, 0xF, 0xF, false);
-    return __builtin_bit_cast(double, (unsigned long long)lo | ((unsigned long long)hi << 32));
-}
-
-// Wave-uniform maximum of a double: DPP row rotations (8, 4, 2, 1) leave each
-// 16-lane row's maximum in all its lanes, then the four rows are combined.
-template <class T>
-__device__ __forceinline__ T wave_max_dpp(T v) {
-    v = fmax(v, dpp_d<0x128>(v));
-    v = fmax(v, dpp_d<0x124>(v));
-    v = fmax(v, dpp_d<0x122>(v));
-    v = fmax(v, dpp_d<0x121>(v));
-    return fmax(fmax(readlane_d(v, 0), readlane_d(v, 16)), fmax(readlane_d(v, 32), readlane_d(v, 48)));
-}
-
-// Pivot-search key: |value| with its 6 lowest significand bits replaced by
-// 63 - lane, so one max reduction yields the pivot row (ties to the lower row).
-__device__ __forceinline__ double pivot_key(double mag, int lane) {
-    const unsigned long long b = __builtin_bit_cast(unsigned long long, mag);
-    return __builtin_bit_cast(double, (b & ~63ull) | (unsigned long long)(63 - lane));
-}
-__device__ __forceinline__ float pivot_key(float mag, int lane) {
-    const unsigned b = __builtin_bit_cast(unsigned, mag);
-    return __builtin_bit_cast(float, (b & ~63u) | (unsigned)(63 - lane));
-}
-__device__ __forceinline__ int key_lane(double key) {
-    return 63 - int(__builtin_bit_cast(unsigned long long, key) & 63ull);
-}
-__device__ __forceinline__ int key_lane(float key) { return 63 - int(__builtin_bit_cast(unsigned, key) & 63u); }
-
 // LDS staging block k viewed as elements of T (row stride LS).
 template <class T>
 __device__ __forceinline__ T* lds_block(LuLds& L, int k) { return reinterpret_cast<T*>(L.B[k]); }
