@@ -767,8 +767,8 @@ __device__ __noinline__ int block_lu(T* __restrict__ Pr_, T* __restrict__ Pi_, i
 }
 
 // Single right-hand side solve with the block factors, latency-shaped: each
-// thread (row i = tid / 8, part q = tid % 8) loads a block-row's factor
-// entries block by block (8 per thread, unrolled and predicated), issuing the
+// thread (row i = tid / 4, part q = tid % 4) loads a block-row's factor
+// entries block by block (16 per thread, unrolled and predicated), issuing the
 // next block's loads before contracting the current one, so a block-row step
 // waits for about one memory latency; the last solution blocks live in an LDS
 // ring (4 x 64, re and im) instead of being re-read from memory. ring: LDS of
@@ -893,133 +893,122 @@ __device__ __noinline__ void block_solve1(const T* __restrict__ Pr_, const T* __
     }
 }
 
-// Solve E x = b in place with the block factors. Real: NR right-hand sides
-// x[0..NR-1]; complex: one right-hand side (xr[0], xi[0]).
-// Thread (i, q): row i = tid / 4 of the current block, part q = tid % 4.
-template <bool CPLX, int NR, class T = double>
-__device__ void block_solve(const T* __restrict__ Pr, const T* __restrict__ Pi, int ns, int m,
-                            double* const* xr, double* const* xi, double* sT /* LDS [2 * NR * MAX_BLOCK] */) {
+// NR (<= 3) right-hand sides x0..x{NR-1} with the FP64 real factors (the
+// GMRES preconditioner), shaped like block_solve1: a block row's factor
+// entries are loaded once, one block ahead, and contracted with every
+// right-hand side; each right-hand side keeps its last solution blocks in its
+// own LDS ring. ring: LDS of at least 5 * NR * 64 doubles.
+template <int NR>
+__device__ __noinline__ void block_solve_multi(const double* __restrict__ P_, int ns, int m, double* x0_,
+                                               double* x1_, double* x2_, double* ring) {
+    static_assert(NR >= 1 && NR <= 3, "one to three right-hand sides");
+    GAS const double* const P = uniform(gas(P_));
+    GAS double* x[3] = {uniform(gas(x0_)), NR > 1 ? uniform(gas(x1_)) : nullptr,
+                        NR > 2 ? uniform(gas(x2_)) : nullptr};
+    ns = uniform(ns), m = uniform(m);
     const int ld = 7 * m;
     const long long ps = (long long)m * ld;
-    const int tid = threadIdx.x, i = tid >> 2, q = tid & 3;
+    constexpr int TPR = RNT / MAX_BLOCK;  // threads per row
+    constexpr int U = MAX_BLOCK / TPR;    // columns per thread per block
+    const int tid = threadIdx.x, i = tid / TPR, q = tid % TPR;
     const bool row = i < m;
-    double* sTr = sT;
-    double* sTi = sT + NR * MAX_BLOCK;
-    // forward: y_I = b_I - sum_{J=I-3}^{I-1} L_IJ y_J
-    for (int I = 0; I < ns; ++I) {
-        const T* PIr = Pr + I * ps;
-        const T* PIi = Pi + I * ps;
-        double ar[NR], ai[NR];
+    double* rg = ring;                // ring of right-hand side v: rg + v * 256, slot J & 3
+    double* tv = ring + NR * 4 * 64;  // temporaries: tv + v * 64
+    double a0[U], a1[U];
+    GAS const double* pr = nullptr;
+    auto load = [&](int col, bool ok, double* a) {
 #pragma unroll
-        for (int v = 0; v < NR; ++v) ar[v] = 0.0, ai[v] = 0.0;
-        if (row) {
-            const int d0 = I >= 3 ? 0 : 3 - I;
-            for (int c = d0 * m + q; c < 3 * m; c += 4) {
-                const int J = I - 3 + c / m, k = c - (c / m) * m;
-                const double lr = PIr[(long long)i * ld + c];
-                if (CPLX) {
-                    const double li = PIi[(long long)i * ld + c];
-                    const double yr = xr[0][J * m + k], yi = xi[0][J * m + k];
-                    ar[0] += lr * yr - li * yi;
-                    ai[0] += lr * yi + li * yr;
-                } else {
-#pragma unroll
-                    for (int v = 0; v < NR; ++v) ar[v] += lr * xr[v][J * m + k];
-                }
-            }
+        for (int u = 0; u < U; ++u) {
+            const int k = q + TPR * u;
+            a[u] = (ok && row && k < m) ? pr[col + k] : 0.0;
         }
+    };
+    auto contract = [&](const double* a, int slot, double* acc) {
 #pragma unroll
         for (int v = 0; v < NR; ++v) {
-            ar[v] += __shfl_xor(ar[v], 1, 64);
-            ar[v] += __shfl_xor(ar[v], 2, 64);
-            if (CPLX) {
-                ai[v] += __shfl_xor(ai[v], 1, 64);
-                ai[v] += __shfl_xor(ai[v], 2, 64);
+            const double* y = rg + v * 256 + slot * 64;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = q + TPR * u;
+                if (k < m) acc[v] += a[u] * y[k];
             }
         }
+    };
+    auto reduce4 = [&](double* acc) {
+#pragma unroll
+        for (int v = 0; v < NR; ++v)
+#pragma unroll
+            for (int o = 1; o < TPR; o <<= 1) acc[v] += __shfl_xor(acc[v], o, 64);
+    };
+    // ring slots of blocks left of the matrix are contracted with zero factor
+    // entries before this solve writes them: start from zeros
+    __syncthreads();
+    for (int e = tid; e < NR * 4 * 64; e += RNT) ring[e] = 0.0;
+    __syncthreads();
+    // forward: y_I = b_I - sum_{d=0..2} L_{I, I-3+d} y_{I-3+d}
+    for (int I = 0; I < ns; ++I) {
+        pr = P + I * ps + (long long)i * ld;
+        const int d0 = I >= 3 ? 0 : 3 - I;
+        double acc[NR];
+#pragma unroll
+        for (int v = 0; v < NR; ++v) acc[v] = 0.0;
+        load(0, d0 <= 0, a0);
+        load(m, d0 <= 1, a1);
+        contract(a0, (I + 1) & 3, acc);
+        load(2 * m, d0 <= 2, a0);
+        contract(a1, (I + 2) & 3, acc);
+        contract(a0, (I + 3) & 3, acc);
+        reduce4(acc);
         if (row && q == 0) {
 #pragma unroll
             for (int v = 0; v < NR; ++v) {
-                xr[v][I * m + i] -= ar[v];
-                if (CPLX) xi[v][I * m + i] -= ai[v];
+                const double y = x[v][I * m + i] - acc[v];
+                rg[v * 256 + (I & 3) * 64 + i] = y;
+                x[v][I * m + i] = y;
             }
         }
         __syncthreads();
     }
-    // backward: x_I = Ainv_I (y_I - sum_{K=I+1}^{I+3} U_IK x_K)
+    // backward: x_I = Ainv_I (y_I - sum_{d=0..2} U_{I, I+1+d} x_{I+1+d})
     for (int I = ns - 1; I >= 0; --I) {
-        const T* PIr = Pr + I * ps;
-        const T* PIi = Pi + I * ps;
-        double ar[NR], ai[NR];
+        pr = P + I * ps + (long long)i * ld;
+        const int nK = (ns - 1 - I) < 3 ? (ns - 1 - I) : 3;
+        double acc[NR];
 #pragma unroll
-        for (int v = 0; v < NR; ++v) ar[v] = 0.0, ai[v] = 0.0;
-        if (row) {
-            const int cmax = (ns - 1 - I < 3 ? ns - 1 - I : 3) * m;
-            for (int c = q; c < cmax; c += 4) {
-                const int K = I + 1 + c / m, k = c - (c / m) * m;
-                const double ur = PIr[(long long)i * ld + 4 * m + c];
-                if (CPLX) {
-                    const double ui = PIi[(long long)i * ld + 4 * m + c];
-                    const double yr = xr[0][K * m + k], yi = xi[0][K * m + k];
-                    ar[0] += ur * yr - ui * yi;
-                    ai[0] += ur * yi + ui * yr;
-                } else {
-#pragma unroll
-                    for (int v = 0; v < NR; ++v) ar[v] += ur * xr[v][K * m + k];
-                }
-            }
-        }
-#pragma unroll
-        for (int v = 0; v < NR; ++v) {
-            ar[v] += __shfl_xor(ar[v], 1, 64);
-            ar[v] += __shfl_xor(ar[v], 2, 64);
-            if (CPLX) {
-                ai[v] += __shfl_xor(ai[v], 1, 64);
-                ai[v] += __shfl_xor(ai[v], 2, 64);
-            }
-        }
+        for (int v = 0; v < NR; ++v) acc[v] = 0.0;
+        load(4 * m, nK > 0, a0);
+        load(5 * m, nK > 1, a1);
+        contract(a0, (I + 1) & 3, acc);
+        load(6 * m, nK > 2, a0);
+        contract(a1, (I + 2) & 3, acc);
+        load(3 * m, true, a1);  // Ainv_I row, used after the barrier
+        contract(a0, (I + 3) & 3, acc);
+        reduce4(acc);
         if (row && q == 0) {
 #pragma unroll
-            for (int v = 0; v < NR; ++v) {
-                sTr[v * MAX_BLOCK + i] = xr[v][I * m + i] - ar[v];
-                if (CPLX) sTi[v * MAX_BLOCK + i] = xi[v][I * m + i] - ai[v];
-            }
+            for (int v = 0; v < NR; ++v) tv[v * 64 + i] = x[v][I * m + i] - acc[v];
         }
         __syncthreads();
 #pragma unroll
-        for (int v = 0; v < NR; ++v) ar[v] = 0.0, ai[v] = 0.0;
-        if (row) {
-            for (int k = q; k < m; k += 4) {
-                const double gr = PIr[(long long)i * ld + 3 * m + k];
-                if (CPLX) {
-                    const double gi = PIi[(long long)i * ld + 3 * m + k];
-                    const double tr = sTr[k], ti = sTi[k];
-                    ar[0] += gr * tr - gi * ti;
-                    ai[0] += gr * ti + gi * tr;
-                } else {
-#pragma unroll
-                    for (int v = 0; v < NR; ++v) ar[v] += gr * sTr[v * MAX_BLOCK + k];
-                }
-            }
-        }
-#pragma unroll
         for (int v = 0; v < NR; ++v) {
-            ar[v] += __shfl_xor(ar[v], 1, 64);
-            ar[v] += __shfl_xor(ar[v], 2, 64);
-            if (CPLX) {
-                ai[v] += __shfl_xor(ai[v], 1, 64);
-                ai[v] += __shfl_xor(ai[v], 2, 64);
+            acc[v] = 0.0;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = q + TPR * u;
+                if (k < m) acc[v] += a1[u] * tv[v * 64 + k];
             }
         }
+        reduce4(acc);
         if (row && q == 0) {
 #pragma unroll
             for (int v = 0; v < NR; ++v) {
-                xr[v][I * m + i] = ar[v];
-                if (CPLX) xi[v][I * m + i] = ai[v];
+                x[v][I * m + i] = acc[v];
+                rg[v * 256 + (I & 3) * 64 + i] = acc[v];
             }
         }
         __syncthreads();
     }
 }
+
 
 }  // namespace fpe

@@ -60,7 +60,6 @@ struct SlotLayout {
 struct RadauLds {
     LuLds lu;
     double red[8 * RNW];
-    double sT[2 * 3 * MAX_BLOCK];
     double H[(GM_K + 1) * GM_K];
     double cs[GM_K + 1], sn[GM_K + 1], gs[GM_K + 1], gy[GM_K];
     int inst;
@@ -407,8 +406,7 @@ __global__ void __launch_bounds__(RNT, RADAU_WG_PER_CU * RNW / 4) k_radau(const 
                     lap(tk_solve);
                 } else {
                     // dF1 and the preconditioned GMRES right-hand side in one pass
-                    double* xr3[3] = {vv(S_DF1), vv(S_VRE0), vv(S_VIM0)};
-                    block_solve<false, 3, double>(P, nullptr, ns, m, xr3, nullptr, S.sT);
+                    block_solve_multi<3>(P, ns, m, vv(S_DF1), vv(S_VRE0), vv(S_VIM0), ring);
                     // ---- GMRES(10), radau.mojo:916-1146
                     double bsq = 0.0;
                     for (int r = tid; r < N; r += RNT) {
@@ -432,8 +430,7 @@ __global__ void __launch_bounds__(RNT, RADAU_WG_PER_CU * RNW / 4) k_radau(const 
                             gmres_mv(Ms, Ks, ns, nv, ALPH, BETA, h, vv(S_VRE0 + j), vv(S_VIM0 + j), vv(S_WRE),
                                      vv(S_WIM));
                             __syncthreads();
-                            double* xw[2] = {vv(S_WRE), vv(S_WIM)};
-                            block_solve<false, 2, double>(P, nullptr, ns, m, xw, nullptr, S.sT);
+                            block_solve_multi<2>(P, ns, m, vv(S_WRE), vv(S_WIM), nullptr, ring);
                             for (int k = 0; k <= j; ++k) {
                                 double hv = 0.0;
                                 for (int r = tid; r < N; r += RNT)
