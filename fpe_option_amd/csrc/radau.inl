@@ -70,15 +70,15 @@ __device__ __forceinline__ void stencil_mv(const double* __restrict__ S, int ns,
                                            double* y) {
     const int N = ns * nv;
     for (int r = threadIdx.x; r < N; r += RNT) {
-        const int I = r / nv, i = r - I * nv;
         double acc = 0.0;
+#pragma unroll 1
         for (int dJ = -3; dJ <= 3; ++dJ) {
-            const int J = I + dJ;
-            if (J < 0 || J >= ns) continue;
+#pragma unroll
             for (int dj = -3; dj <= 3; ++dj) {
-                const int jj = i + dj;
-                if (jj < 0 || jj >= nv) continue;
-                acc += S[(long long)((dJ + 3) * 7 + dj + 3) * N + r] * x[J * nv + jj];
+                // entries of missing neighbours are stored as exact zeros: a clamped
+                // column keeps every load in range and those terms add zero
+                const int c = min(max(r + dJ * nv + dj, 0), N - 1);
+                acc += S[(long long)((dJ + 3) * 7 + dj + 3) * N + r] * x[c];
             }
         }
         y[r] = acc;
@@ -92,16 +92,15 @@ __device__ void newton_mv(const double* __restrict__ Ms, const double* __restric
     auto v = [=](int k) { return vb + (long long)k * N; };
     const double *Z1 = v(S_Z1), *Z2 = v(S_Z2), *Z3 = v(S_Z3), *F1 = v(S_F1), *F2 = v(S_F2), *F3 = v(S_F3);
     for (int r = threadIdx.x; r < N; r += RNT) {
-        const int I = r / nv, i = r - I * nv;
         double k1 = 0, k2 = 0, k3 = 0, m1 = 0, m2 = 0, m3 = 0;
+#pragma unroll 1
         for (int dJ = -3; dJ <= 3; ++dJ) {
-            const int J = I + dJ;
-            if (J < 0 || J >= ns) continue;
+#pragma unroll
             for (int dj = -3; dj <= 3; ++dj) {
-                const int jj = i + dj;
-                if (jj < 0 || jj >= nv) continue;
+                // entries of missing neighbours are stored as exact zeros: a clamped
+                // column keeps every load in range and those terms add zero
                 const long long e = (long long)((dJ + 3) * 7 + dj + 3) * N + r;
-                const int c = J * nv + jj;
+                const int c = min(max(r + dJ * nv + dj, 0), N - 1);
                 const double kv = Ks[e], mv = Ms[e];
                 k1 += kv * Z1[c], k2 += kv * Z2[c], k3 += kv * Z3[c];
                 m1 += mv * F1[c], m2 += mv * F2[c], m3 += mv * F3[c];
@@ -118,16 +117,15 @@ __device__ void gmres_mv(const double* __restrict__ Ms, const double* __restrict
                          double beta, double h, const double* vr, const double* vi, double* wr, double* wi) {
     const int N = ns * nv;
     for (int r = threadIdx.x; r < N; r += RNT) {
-        const int I = r / nv, i = r - I * nv;
         double er = 0, ei = 0, mr = 0, mi = 0;
+#pragma unroll 1
         for (int dJ = -3; dJ <= 3; ++dJ) {
-            const int J = I + dJ;
-            if (J < 0 || J >= ns) continue;
+#pragma unroll
             for (int dj = -3; dj <= 3; ++dj) {
-                const int jj = i + dj;
-                if (jj < 0 || jj >= nv) continue;
+                // entries of missing neighbours are stored as exact zeros: a clamped
+                // column keeps every load in range and those terms add zero
                 const long long e = (long long)((dJ + 3) * 7 + dj + 3) * N + r;
-                const int c = J * nv + jj;
+                const int c = min(max(r + dJ * nv + dj, 0), N - 1);
                 const double mv = Ms[e];
                 const double ed = alph * mv + h * Ks[e];
                 er += ed * vr[c];
@@ -148,16 +146,15 @@ __device__ void stage_residual(const double* __restrict__ Ms, const double* __re
                                const double* bi, double* rr, double* ri, bool cplx) {
     const int N = ns * nv;
     for (int r = threadIdx.x; r < N; r += RNT) {
-        const int I = r / nv, i = r - I * nv;
         double er = 0, ei = 0, mr = 0, mi = 0;
+#pragma unroll 1
         for (int dJ = -3; dJ <= 3; ++dJ) {
-            const int J = I + dJ;
-            if (J < 0 || J >= ns) continue;
+#pragma unroll
             for (int dj = -3; dj <= 3; ++dj) {
-                const int jj = i + dj;
-                if (jj < 0 || jj >= nv) continue;
+                // entries of missing neighbours are stored as exact zeros: a clamped
+                // column keeps every load in range and those terms add zero
                 const long long e = (long long)((dJ + 3) * 7 + dj + 3) * N + r;
-                const int c = J * nv + jj;
+                const int c = min(max(r + dJ * nv + dj, 0), N - 1);
                 const double mv = Ms[e];
                 const double ed = a * mv + h * Ks[e];
                 er += ed * xr[c];
