@@ -617,12 +617,10 @@ __device__ void assemble_E(const double* __restrict__ Ms, const double* __restri
             if (J < 0 || J >= ns) continue;
             double mv[7], kv[7];
 #pragma unroll
-            for (int q = 0; q < 7; ++q) {
-                const int j = i + q - 3;
-                const bool ok = j >= 0 && j < m;
+            for (int q = 0; q < 7; ++q) {  // every plane has an entry per row (zero for missing neighbours)
                 const long long o = (long long)(d * 7 + q) * N + row;
-                mv[q] = ok ? Ms[o] : 0.0;
-                kv[q] = ok ? Ks[o] : 0.0;
+                mv[q] = Ms[o];
+                kv[q] = Ks[o];
             }
 #pragma unroll
             for (int q = 0; q < 7; ++q) {
