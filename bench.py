@@ -112,7 +112,7 @@ def surrogate_leg(plain_value, n=1_000_000, audit_frac=0.01, n_out=256, seed=202
             "audit": {"n": int(len(pick)), "max_abs_dprice": float(np.max(d)) if len(d) else None,
                       "p99_abs_dprice": float(np.percentile(d, 99)) if len(d) else None,
                       "mean_abs_dprice": float(np.mean(d)) if len(d) else None,
-                      "judge": "plain FP64 solve, same 80x40 grid"},
+                      "checked_against": "plain FP64 solve, same 80x40 grid"},
             "model": {"samples": int(sur.info["meta_samples"]), "val_max_abs_dprice":
                       float(sur.info["meta_val_max_abs_dprice"])},
             "workload": f"{n} seeded European calls/puts from the training box (kappa, theta, sigma, rho, V0, r, T), "

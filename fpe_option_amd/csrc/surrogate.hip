@@ -2,7 +2,7 @@
 // FP32 accumulation (forward, backward, Adam, loss), and pricing from POD
 // coefficients of the terminal density (coefficients -> s-marginal -> payoff).
 // Nothing here has a counterpart in the reference engine; the plain
-// Fokker-Planck solve is the surrogate's fallback and its judge.
+// Fokker-Planck solve is the surrogate's fallback and its accuracy reference.
 //
 // Lane layouts (checked on gfx950 with an in-bounds probe):
 //   v_mfma_f32_16x16x16_bf16: a[j] = A[l%16][4(l/16)+j], b[j] = B[4(l/16)+j][l%16],

@@ -1,5 +1,5 @@
 """POD + MLP surrogate for European prices on a fixed grid, with the plain
-Fokker-Planck solve as its fallback and its judge.
+Fokker-Planck solve as its fallback and its accuracy reference.
 
 Model: the terminal density q(T) of a European solve on a fixed span-mode grid
 (n_s = 80, n_v = 40, s in [0, 180], v in [0, 1.5], S0 = 60, v-knots clustered
@@ -112,7 +112,7 @@ def pricing_operator():
 
 
 def solver_prices(P, strike, otype):
-    """The plain FP64 solve on the same fixed grid (the surrogate's judge)."""
+    """The plain FP64 solve on the same fixed grid (the surrogate's accuracy reference)."""
     from . import api
     cols = {name: P[:, j] for j, name in enumerate(PARAMS)}
     cols.update(GRID)

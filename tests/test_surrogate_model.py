@@ -1,4 +1,4 @@
-"""The trained surrogate (fpe_option_amd/surrogate_80x40.npz) against its judge,
+"""The trained surrogate (fpe_option_amd/surrogate_80x40.npz) against its accuracy reference,
 the plain FP64 solve on the same 80x40 grid, on fresh seeded parameter sets
 that were not in the training data; and what the network adds over the same
 POD pricing without it (every option priced at the mean coefficients)."""
