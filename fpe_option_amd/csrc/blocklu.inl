@@ -135,7 +135,7 @@ __device__ __forceinline__ float recip(float d) {
 
 // Steps k0..k0+nb-1 of the blocked Gauss-Jordan inverse on this wave's 16
 // columns c = 16 w + lane % 16. Pivot rows follow the step-by-step recurrence
-// (every lane of a 16-lane group repeats it for its column):
+// (the four lanes of a column each own two of its pivot rows):
 //     y_s = x_{p_s}, x_{p_s} <- y_s / piv_s, x_{p_t} <- x_{p_t} + U[p_t][s] y_s (t != s),
 // where U[.][s] is panel column s right after step s (-f / piv_s off the pivot
 // row); every other row takes x_i <- x_i + sum_s U[i][s] y_s as a rank-nb MFMA
@@ -153,40 +153,46 @@ __device__ __forceinline__ void apply_panel(LuLds& L, int m, int k0, int nb) {
     T* B1 = lds_block<T>(L, 1);
     const T* Ur = reinterpret_cast<const T*>(L.U[0]);
     const T* Ui = reinterpret_cast<const T*>(L.U[1]);
-    int p[GJ_PW];
-    T zr[GJ_PW], zi[GJ_PW], yr[GJ_PW], yi[GJ_PW];
+    // lane group g = lane / 16 owns pivot rows t = g + 4 h (h < GJ_PW / 4) of its
+    // column's recurrence; y_s comes from the owner group of s by a lane shuffle
+    constexpr int H = GJ_PW / 4;
+    int po[H];
+    T zr[H], zi[H], br[H], bi[H];
 #pragma unroll
-    for (int t = 0; t < GJ_PW; ++t) {
-        p[t] = t < nb ? L.pv[t] : 0;
-        zr[t] = B0[p[t] * LS + c];
-        zi[t] = CPLX ? B1[p[t] * LS + c] : T(0);
-        yr[t] = yi[t] = T(0);
+    for (int h = 0; h < H; ++h) {
+        const int t = 4 * h + l4;
+        int pt = 0;
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+            if (g == l4 && 4 * h + g < nb) pt = L.pv[4 * h + g];
+        po[h] = t < nb ? pt : 0;
+        zr[h] = B0[po[h] * LS + c];
+        zi[h] = CPLX ? B1[po[h] * LS + c] : T(0);
+        br[h] = bi[h] = T(0);
     }
 #pragma unroll
     for (int s = 0; s < GJ_PW; ++s) {
         if (s < nb) {
-            const T rr = T(L.ip[0][s]), ri = CPLX ? T(L.ip[1][s]) : T(0);
-            yr[s] = zr[s], yi[s] = zi[s];
-            zr[s] = CPLX ? yr[s] * rr - yi[s] * ri : yr[s] * rr;
-            if (CPLX) zi[s] = yr[s] * ri + yi[s] * rr;
+            const int og = s & 3, oh = s >> 2;  // owner lane group and slot of row p_s
+            const int src = l16 + 16 * og;
+            const T yr = __shfl(zr[oh], src, 64);
+            const T yi = CPLX ? __shfl(zi[oh], src, 64) : T(0);
+            if (l4 == og) {  // owner: x_{p_s} <- y_s / piv_s; y_s is also its MFMA B operand
+                const T rr = T(L.ip[0][s]), ri = CPLX ? T(L.ip[1][s]) : T(0);
+                br[oh] = yr, bi[oh] = yi;
+                zr[oh] = CPLX ? yr * rr - yi * ri : yr * rr;
+                if (CPLX) zi[oh] = yr * ri + yi * rr;
+            }
 #pragma unroll
-            for (int t = 0; t < GJ_PW; ++t) {
+            for (int h = 0; h < H; ++h) {
+                const int t = 4 * h + l4;
                 if (t != s && t < nb) {
-                    const T ur = Ur[s * GJ_US + p[t]], ui = CPLX ? Ui[s * GJ_US + p[t]] : T(0);
-                    zr[t] += CPLX ? ur * yr[s] - ui * yi[s] : ur * yr[s];
-                    if (CPLX) zi[t] += ur * yi[s] + ui * yr[s];
+                    const T ur = Ur[s * GJ_US + po[h]], ui = CPLX ? Ui[s * GJ_US + po[h]] : T(0);
+                    zr[h] += CPLX ? ur * yr - ui * yi : ur * yr;
+                    if (CPLX) zi[h] += ur * yi + ui * yr;
                 }
             }
         }
-    }
-    // B operands: y_{4 kk + lane / 16} of column c
-    T br[GJ_PW / 4], bi[GJ_PW / 4];
-#pragma unroll
-    for (int kk = 0; kk < GJ_PW / 4; ++kk) {
-        br[kk] = bi[kk] = T(0);
-#pragma unroll
-        for (int t = 0; t < 4; ++t)
-            if (t == l4) br[kk] = yr[4 * kk + t], bi[kk] = yi[4 * kk + t];
     }
     const int tm = (m + 15) >> 4;
     for (int tt = 0; tt < tm; ++tt) {
@@ -212,17 +218,12 @@ __device__ __forceinline__ void apply_panel(LuLds& L, int m, int k0, int nb) {
             if (CPLX) B1[r * LS + c] = acci[j];
         }
     }
-    // pivot rows last (lane group l4 writes t = l4, l4 + 4, ...; t >= nb to padding)
+    // pivot rows last (lane group l4 writes its rows t = l4 + 4 h; t >= nb to padding)
 #pragma unroll
-    for (int h = 0; h < GJ_PW / 4; ++h) {
-        T vr = T(0), vi = T(0);
-        int pr = 0;
-#pragma unroll
-        for (int t = 0; t < 4; ++t)
-            if (t == l4) vr = zr[4 * h + t], vi = zi[4 * h + t], pr = p[4 * h + t];
-        const int o = (4 * h + l4 < nb) ? pr * LS + c : MAX_BLOCK;  // row 0, padding column
-        B0[o] = vr;
-        if (CPLX) B1[o] = vi;
+    for (int h = 0; h < H; ++h) {
+        const int o = (4 * h + l4 < nb) ? po[h] * LS + c : MAX_BLOCK;  // row 0, padding column
+        B0[o] = zr[h];
+        if (CPLX) B1[o] = zi[h];
     }
 }
 
