@@ -507,22 +507,32 @@ __device__ void schur_pass(GAS T* __restrict__ Pr, GAS T* __restrict__ Pi, long 
                            int kb0, int nkb, const LuLds& L) {
     typedef typename Mf<T>::acc A4;
     constexpr int TG = CPLX ? 2 : 4;  // column tiles per accumulator group
+    constexpr bool PF = !CPLX;        // next-strip prefetch (complex: no registers to spare)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, l4 = lane >> 4;
     const int M = nU * m, tm = (M + 15) >> 4, tk = (m + 3) >> 2, tn = (m + 15) >> 4;
-    for (int t = wave; t < tm; t += RNW) {
-        const int r0 = t << 4, ra = r0 + l16;
+    // L strip of 16 rows (t = strip) into registers; the next strip of this wave
+    // is fetched while the current one's MFMA chains run
+    auto fetch = [&](int t, T* fr, T* fi) {
+        const int ra = (t << 4) + l16;
         const bool va = ra < M;
         int dJa;
         const long long oa = va ? stacked_row(ps, ld, I, m, ra, dJa) + (3 - dJa) * m : 0;
-        T ar[MAX_BLOCK / 4], ai[MAX_BLOCK / 4];
 #pragma unroll
         for (int s = 0; s < MAX_BLOCK / 4; ++s) {
             const int k = 4 * s + l4;
             const bool ok = s < tk && va && k < m;
-            ar[s] = ok ? Pr[oa + k] : T(0);
-            ai[s] = (CPLX && ok) ? Pi[oa + k] : T(0);
+            fr[s] = ok ? Pr[oa + k] : T(0);
+            fi[s] = (CPLX && ok) ? Pi[oa + k] : T(0);
         }
+    };
+    T ar[MAX_BLOCK / 4], ai[MAX_BLOCK / 4], nr[MAX_BLOCK / 4], ni[MAX_BLOCK / 4];
+#pragma unroll
+    for (int s = 0; s < MAX_BLOCK / 4; ++s) ar[s] = ai[s] = nr[s] = ni[s] = T(0);
+    if (wave < tm) fetch(wave, ar, ai);
+    for (int t = wave; t < tm; t += RNW) {
+        const int r0 = t << 4;
+        if (PF && t + RNW < tm) fetch(t + RNW, nr, ni);
         long long oc[4];
         int dj[4];
         bool vc[4];
@@ -584,6 +594,12 @@ __device__ void schur_pass(GAS T* __restrict__ Pr, GAS T* __restrict__ Pi, long 
                     }
                 }
             }
+        }
+        if (PF) {
+#pragma unroll
+            for (int s2 = 0; s2 < MAX_BLOCK / 4; ++s2) ar[s2] = nr[s2], ai[s2] = ni[s2];
+        } else if (t + RNW < tm) {
+            fetch(t + RNW, ar, ai);
         }
     }
 }
