@@ -26,7 +26,7 @@ constexpr int RNT = 256;
 constexpr int RNW = RNT / 64;
 constexpr int RADAU_WG_PER_CU = 2;
 constexpr int LS = MAX_BLOCK + 1;  // LDS row stride of an m x m block (column MAX_BLOCK is padding)
-constexpr int GJ_PW = 8;           // pivot steps per Gauss-Jordan panel
+constexpr int GJ_PW = 4;           // pivot steps per Gauss-Jordan panel (a multiple of 4)
 constexpr int GJ_US = 65;          // LDS stride of a panel column (odd: the MFMA operand reads of
                                    // four lane groups fall in different banks)
 
