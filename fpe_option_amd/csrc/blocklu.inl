@@ -810,7 +810,19 @@ __device__ __noinline__ void block_solve1(const T* __restrict__ Pr_, const T* __
     double a0[U], b0[U], a1[U], b1[U];
     GAS const T* pr = nullptr;
     GAS const T* pi = nullptr;
-    // factor entries (row i, columns col + q + 4u) of the current block row
+    // factor entries (row i, columns col + q + 4u) of block row I
+    auto load_row = [&](int I, int col, bool ok, double* a, double* b) {
+        GAS const T* qr = Pr + I * ps + (long long)i * ld;
+        GAS const T* qi = CPLX ? Pi + I * ps + (long long)i * ld : nullptr;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = q + TPR * u;
+            const bool v = ok && row && k < m;
+            a[u] = v ? double(qr[col + k]) : 0.0;
+            b[u] = (CPLX && v) ? double(qi[col + k]) : 0.0;
+        }
+    };
+    // ... of the current block row
     auto load = [&](int col, bool ok, double* a, double* b) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -849,18 +861,23 @@ __device__ __noinline__ void block_solve1(const T* __restrict__ Pr_, const T* __
     __syncthreads();
     for (int e = tid; e < 8 * 64; e += RNT) ring[e] = 0.0;
     __syncthreads();
-    // forward: y_I = b_I - sum_{d=0..2} L_{I, I-3+d} y_{I-3+d} (ring slot of block J: J & 3)
+    // forward: y_I = b_I - sum_{d=0..2} L_{I, I-3+d} y_{I-3+d} (ring slot of block J: J & 3);
+    // the first block of row I + 1 is loaded while row I finishes
+    auto d0_of = [&](int I) { return I >= 3 ? 0 : 3 - I; };  // blocks d < d0 lie left of the matrix
+    if (ns > 0) load_row(0, 0, d0_of(0) <= 0, a0, b0);
     for (int I = 0; I < ns; ++I) {
         pr = Pr + I * ps + (long long)i * ld;
         pi = CPLX ? Pi + I * ps + (long long)i * ld : nullptr;
-        const int d0 = I >= 3 ? 0 : 3 - I;  // blocks d < d0 lie left of the matrix
+        const int d0 = d0_of(I);
         double ar = 0.0, ai = 0.0;
-        load(0, d0 <= 0, a0, b0);
         load(m, d0 <= 1, a1, b1);
         contract(a0, b0, rr + ((I + 1) & 3) * 64, ri + ((I + 1) & 3) * 64, ar, ai);
         load(2 * m, d0 <= 2, a0, b0);
         contract(a1, b1, rr + ((I + 2) & 3) * 64, ri + ((I + 2) & 3) * 64, ar, ai);
+        if (I + 1 < ns) load_row(I + 1, 0, d0_of(I + 1) <= 0, a1, b1);
         contract(a0, b0, rr + ((I + 3) & 3) * 64, ri + ((I + 3) & 3) * 64, ar, ai);
+#pragma unroll
+        for (int u = 0; u < U; ++u) a0[u] = a1[u], b0[u] = b1[u];
         reduce4(ar, ai);
         if (row && q == 0) {
             const double vr = xr[I * m + i] - ar;
@@ -874,19 +891,22 @@ __device__ __noinline__ void block_solve1(const T* __restrict__ Pr_, const T* __
         }
         __syncthreads();
     }
-    // backward: x_I = Ainv_I (y_I - sum_{d=0..2} U_{I, I+1+d} x_{I+1+d})
+    // backward: x_I = Ainv_I (y_I - sum_{d=0..2} U_{I, I+1+d} x_{I+1+d}); the first
+    // block of row I - 1 is loaded while row I finishes
+    auto nK_of = [&](int I) { return (ns - 1 - I) < 3 ? (ns - 1 - I) : 3; };  // blocks right of the matrix are skipped
+    if (ns > 0) load_row(ns - 1, 4 * m, nK_of(ns - 1) > 0, a0, b0);
     for (int I = ns - 1; I >= 0; --I) {
         pr = Pr + I * ps + (long long)i * ld;
         pi = CPLX ? Pi + I * ps + (long long)i * ld : nullptr;
-        const int nK = (ns - 1 - I) < 3 ? (ns - 1 - I) : 3;  // blocks right of the matrix are skipped
+        const int nK = nK_of(I);
         double ar = 0.0, ai = 0.0;
-        load(4 * m, nK > 0, a0, b0);
         load(5 * m, nK > 1, a1, b1);
         contract(a0, b0, rr + ((I + 1) & 3) * 64, ri + ((I + 1) & 3) * 64, ar, ai);
         load(6 * m, nK > 2, a0, b0);
         contract(a1, b1, rr + ((I + 2) & 3) * 64, ri + ((I + 2) & 3) * 64, ar, ai);
         load(3 * m, true, a1, b1);  // Ainv_I row, used after the barrier
         contract(a0, b0, rr + ((I + 3) & 3) * 64, ri + ((I + 3) & 3) * 64, ar, ai);
+        if (I > 0) load_row(I - 1, 4 * m, nK_of(I - 1) > 0, a0, b0);
         reduce4(ar, ai);
         if (row && q == 0) {
             tr[i] = xr[I * m + i] - ar;
@@ -938,6 +958,14 @@ __device__ __noinline__ void block_solve_multi(const double* __restrict__ P_, in
             a[u] = (ok && row && k < m) ? pr[col + k] : 0.0;
         }
     };
+    auto load_row = [&](int I, int col, bool ok, double* a) {
+        GAS const double* qr = P + I * ps + (long long)i * ld;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = q + TPR * u;
+            a[u] = (ok && row && k < m) ? qr[col + k] : 0.0;
+        }
+    };
     auto contract = [&](const double* a, int slot, double* acc) {
 #pragma unroll
         for (int v = 0; v < NR; ++v) {
@@ -960,19 +988,24 @@ __device__ __noinline__ void block_solve_multi(const double* __restrict__ P_, in
     __syncthreads();
     for (int e = tid; e < NR * 4 * 64; e += RNT) ring[e] = 0.0;
     __syncthreads();
-    // forward: y_I = b_I - sum_{d=0..2} L_{I, I-3+d} y_{I-3+d}
+    // forward: y_I = b_I - sum_{d=0..2} L_{I, I-3+d} y_{I-3+d}; the first block of
+    // row I + 1 is loaded while row I finishes
+    auto d0_of = [&](int I) { return I >= 3 ? 0 : 3 - I; };
+    if (ns > 0) load_row(0, 0, d0_of(0) <= 0, a0);
     for (int I = 0; I < ns; ++I) {
         pr = P + I * ps + (long long)i * ld;
-        const int d0 = I >= 3 ? 0 : 3 - I;
+        const int d0 = d0_of(I);
         double acc[NR];
 #pragma unroll
         for (int v = 0; v < NR; ++v) acc[v] = 0.0;
-        load(0, d0 <= 0, a0);
         load(m, d0 <= 1, a1);
         contract(a0, (I + 1) & 3, acc);
         load(2 * m, d0 <= 2, a0);
         contract(a1, (I + 2) & 3, acc);
+        if (I + 1 < ns) load_row(I + 1, 0, d0_of(I + 1) <= 0, a1);
         contract(a0, (I + 3) & 3, acc);
+#pragma unroll
+        for (int u = 0; u < U; ++u) a0[u] = a1[u];
         reduce4(acc);
         if (row && q == 0) {
 #pragma unroll
@@ -984,20 +1017,23 @@ __device__ __noinline__ void block_solve_multi(const double* __restrict__ P_, in
         }
         __syncthreads();
     }
-    // backward: x_I = Ainv_I (y_I - sum_{d=0..2} U_{I, I+1+d} x_{I+1+d})
+    // backward: x_I = Ainv_I (y_I - sum_{d=0..2} U_{I, I+1+d} x_{I+1+d}); the first
+    // block of row I - 1 is loaded while row I finishes
+    auto nK_of = [&](int I) { return (ns - 1 - I) < 3 ? (ns - 1 - I) : 3; };
+    if (ns > 0) load_row(ns - 1, 4 * m, nK_of(ns - 1) > 0, a0);
     for (int I = ns - 1; I >= 0; --I) {
         pr = P + I * ps + (long long)i * ld;
-        const int nK = (ns - 1 - I) < 3 ? (ns - 1 - I) : 3;
+        const int nK = nK_of(I);
         double acc[NR];
 #pragma unroll
         for (int v = 0; v < NR; ++v) acc[v] = 0.0;
-        load(4 * m, nK > 0, a0);
         load(5 * m, nK > 1, a1);
         contract(a0, (I + 1) & 3, acc);
         load(6 * m, nK > 2, a0);
         contract(a1, (I + 2) & 3, acc);
         load(3 * m, true, a1);  // Ainv_I row, used after the barrier
         contract(a0, (I + 3) & 3, acc);
+        if (I > 0) load_row(I - 1, 4 * m, nK_of(I - 1) > 0, a0);
         reduce4(acc);
         if (row && q == 0) {
 #pragma unroll
