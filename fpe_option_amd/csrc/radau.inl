@@ -403,7 +403,9 @@ __global__ void __launch_bounds__(RNT, RADAU_WG_PER_CU * RNW / 4) k_radau(const 
                     lap(tk_solve);
                 } else {
                     // dF1 and the preconditioned GMRES right-hand side in one pass
+                    tk_mark = wall_clock64();
                     block_solve_multi<3>(P, ns, m, vv(S_DF1), vv(S_VRE0), vv(S_VIM0), ring);
+                    lap(tk_solve);
                     // ---- GMRES(10), radau.mojo:916-1146
                     double bsq = 0.0;
                     for (int r = tid; r < N; r += RNT) {
@@ -424,10 +426,13 @@ __global__ void __launch_bounds__(RNT, RADAU_WG_PER_CU * RNW / 4) k_radau(const 
                         __syncthreads();
                         int j = 0;
                         while (j < GM_K) {
+                            tk_mark = wall_clock64();
                             gmres_mv(Ms, Ks, ns, nv, ALPH, BETA, h, vv(S_VRE0 + j), vv(S_VIM0 + j), vv(S_WRE),
                                      vv(S_WIM));
                             __syncthreads();
+                            lap(tk_mv);
                             block_solve_multi<2>(P, ns, m, vv(S_WRE), vv(S_WIM), nullptr, ring);
+                            lap(tk_solve);
                             for (int k = 0; k <= j; ++k) {
                                 double hv = 0.0;
                                 for (int r = tid; r < N; r += RNT)
