@@ -65,6 +65,23 @@ struct RadauLds {
     int inst;
 };
 
+// Stencil products. Each thread owns one row r and walks the 7 stencil rows
+// dJ = -3..3; for each it gathers the 7-entry column window (plane values and
+// the vector entries they multiply) into registers before using any of it, so
+// all loads of a window are in flight together. The scheduling barrier keeps
+// the compiler from pairing every load with its use, which would serialise one
+// memory latency per stencil entry. Accumulation order is dJ-major, then dj,
+// as in a plain double loop.
+// Entries of missing neighbours are stored as exact zeros: a clamped column
+// keeps every load in range and those terms add zero.
+__device__ __forceinline__ void window_cols(int* c, int r, int dJ, int nv, int N) {
+#pragma unroll
+    for (int q = 0; q < 7; ++q) c[q] = min(max(r + dJ * nv + q - 3, 0), N - 1);
+}
+__device__ __forceinline__ long long plane_at(int dJ, int q, int N, int r) {
+    return (long long)((dJ + 3) * 7 + q) * N + r;
+}
+
 // y_r = sum_e S[e][r] x[col(r,e)] over the 49-point stencil (column order).
 __device__ __forceinline__ void stencil_mv(const double* __restrict__ S, int ns, int nv, const double* x,
                                            double* y) {
@@ -73,13 +90,14 @@ __device__ __forceinline__ void stencil_mv(const double* __restrict__ S, int ns,
         double acc = 0.0;
 #pragma unroll 1
         for (int dJ = -3; dJ <= 3; ++dJ) {
+            int c[7];
+            double s[7], xv[7];
+            window_cols(c, r, dJ, nv, N);
 #pragma unroll
-            for (int dj = -3; dj <= 3; ++dj) {
-                // entries of missing neighbours are stored as exact zeros: a clamped
-                // column keeps every load in range and those terms add zero
-                const int c = min(max(r + dJ * nv + dj, 0), N - 1);
-                acc += S[(long long)((dJ + 3) * 7 + dj + 3) * N + r] * x[c];
-            }
+            for (int q = 0; q < 7; ++q) s[q] = S[plane_at(dJ, q, N, r)], xv[q] = x[c[q]];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int q = 0; q < 7; ++q) acc += s[q] * xv[q];
         }
         y[r] = acc;
     }
@@ -95,16 +113,20 @@ __device__ void newton_mv(const double* __restrict__ Ms, const double* __restric
         double k1 = 0, k2 = 0, k3 = 0, m1 = 0, m2 = 0, m3 = 0;
 #pragma unroll 1
         for (int dJ = -3; dJ <= 3; ++dJ) {
+            int c[7];
+            double s[7], a[7], b[7], d[7];
+            window_cols(c, r, dJ, nv, N);
 #pragma unroll
-            for (int dj = -3; dj <= 3; ++dj) {
-                // entries of missing neighbours are stored as exact zeros: a clamped
-                // column keeps every load in range and those terms add zero
-                const long long e = (long long)((dJ + 3) * 7 + dj + 3) * N + r;
-                const int c = min(max(r + dJ * nv + dj, 0), N - 1);
-                const double kv = Ks[e], mv = Ms[e];
-                k1 += kv * Z1[c], k2 += kv * Z2[c], k3 += kv * Z3[c];
-                m1 += mv * F1[c], m2 += mv * F2[c], m3 += mv * F3[c];
-            }
+            for (int q = 0; q < 7; ++q) s[q] = Ks[plane_at(dJ, q, N, r)], a[q] = Z1[c[q]], b[q] = Z2[c[q]], d[q] = Z3[c[q]];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int q = 0; q < 7; ++q) k1 += s[q] * a[q], k2 += s[q] * b[q], k3 += s[q] * d[q];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int q = 0; q < 7; ++q) s[q] = Ms[plane_at(dJ, q, N, r)], a[q] = F1[c[q]], b[q] = F2[c[q]], d[q] = F3[c[q]];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int q = 0; q < 7; ++q) m1 += s[q] * a[q], m2 += s[q] * b[q], m3 += s[q] * d[q];
         }
         v(S_KZ1)[r] = k1, v(S_KZ2)[r] = k2, v(S_KZ3)[r] = k3;
         v(S_MF1)[r] = m1, v(S_MF2)[r] = m2, v(S_MF3)[r] = m3;
@@ -120,18 +142,22 @@ __device__ void gmres_mv(const double* __restrict__ Ms, const double* __restrict
         double er = 0, ei = 0, mr = 0, mi = 0;
 #pragma unroll 1
         for (int dJ = -3; dJ <= 3; ++dJ) {
+            int c[7];
+            double mv[7], kv[7], xr[7], xi[7];
+            window_cols(c, r, dJ, nv, N);
 #pragma unroll
-            for (int dj = -3; dj <= 3; ++dj) {
-                // entries of missing neighbours are stored as exact zeros: a clamped
-                // column keeps every load in range and those terms add zero
-                const long long e = (long long)((dJ + 3) * 7 + dj + 3) * N + r;
-                const int c = min(max(r + dJ * nv + dj, 0), N - 1);
-                const double mv = Ms[e];
-                const double ed = alph * mv + h * Ks[e];
-                er += ed * vr[c];
-                ei += ed * vi[c];
-                mi += mv * vi[c];
-                mr += mv * vr[c];
+            for (int q = 0; q < 7; ++q) {
+                const long long e = plane_at(dJ, q, N, r);
+                mv[q] = Ms[e], kv[q] = Ks[e], xr[q] = vr[c[q]], xi[q] = vi[c[q]];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int q = 0; q < 7; ++q) {
+                const double ed = alph * mv[q] + h * kv[q];
+                er += ed * xr[q];
+                ei += ed * xi[q];
+                mi += mv[q] * xi[q];
+                mr += mv[q] * xr[q];
             }
         }
         wr[r] = er - beta * mi;
@@ -140,32 +166,38 @@ __device__ void gmres_mv(const double* __restrict__ Ms, const double* __restrict
 }
 
 // Residual of a stage system in FP64 from the stencils: r = b - E x with
-// E = (a + i beta) M + h K (beta = 0 and imaginary parts unused when !cplx).
+// E = (a + i beta) M + h K (beta and the imaginary parts unused when !CPLX).
+template <bool CPLX>
 __device__ void stage_residual(const double* __restrict__ Ms, const double* __restrict__ Ks, int ns, int nv, double a,
                                double beta, double h, const double* xr, const double* xi, const double* br,
-                               const double* bi, double* rr, double* ri, bool cplx) {
+                               const double* bi, double* rr, double* ri) {
     const int N = ns * nv;
     for (int r = threadIdx.x; r < N; r += RNT) {
         double er = 0, ei = 0, mr = 0, mi = 0;
 #pragma unroll 1
         for (int dJ = -3; dJ <= 3; ++dJ) {
+            int c[7];
+            double mv[7], kv[7], yr[7], yi[7];
+            window_cols(c, r, dJ, nv, N);
 #pragma unroll
-            for (int dj = -3; dj <= 3; ++dj) {
-                // entries of missing neighbours are stored as exact zeros: a clamped
-                // column keeps every load in range and those terms add zero
-                const long long e = (long long)((dJ + 3) * 7 + dj + 3) * N + r;
-                const int c = min(max(r + dJ * nv + dj, 0), N - 1);
-                const double mv = Ms[e];
-                const double ed = a * mv + h * Ks[e];
-                er += ed * xr[c];
-                if (cplx) {
-                    ei += ed * xi[c];
-                    mr += mv * xr[c];
-                    mi += mv * xi[c];
+            for (int q = 0; q < 7; ++q) {
+                const long long e = plane_at(dJ, q, N, r);
+                mv[q] = Ms[e], kv[q] = Ks[e], yr[q] = xr[c[q]];
+                if (CPLX) yi[q] = xi[c[q]];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int q = 0; q < 7; ++q) {
+                const double ed = a * mv[q] + h * kv[q];
+                er += ed * yr[q];
+                if (CPLX) {
+                    ei += ed * yi[q];
+                    mr += mv[q] * yr[q];
+                    mi += mv[q] * yi[q];
                 }
             }
         }
-        if (cplx) {
+        if (CPLX) {
             rr[r] = br[r] - (er - beta * mi);
             ri[r] = bi[r] - (ei + beta * mr);
         } else {
@@ -269,7 +301,7 @@ __global__ void __launch_bounds__(RNT, RADAU_WG_PER_CU * RNW / 4) k_radau(const 
             bn = block_max_w<RNW>(bn, S.red);
             block_solve1<false, float>(Pf, nullptr, ns, m, x, nullptr, ring);
             for (int it = 0; it < REFINE_MAX; ++it) {
-                stage_residual(Ms, Ks, ns, nv, U1, 0.0, h_lu, x, nullptr, b, nullptr, tv, nullptr, false);
+                stage_residual<false>(Ms, Ks, ns, nv, U1, 0.0, h_lu, x, nullptr, b, nullptr, tv, nullptr);
                 double rn = 0.0;
                 for (int r = tid; r < N; r += RNT) rn = fmax(rn, fabs(tv[r]));
                 rn = block_max_w<RNW>(rn, S.red);
@@ -297,7 +329,7 @@ __global__ void __launch_bounds__(RNT, RADAU_WG_PER_CU * RNW / 4) k_radau(const 
             bn = block_max_w<RNW>(bn, S.red);
             block_solve1<true, float>(Crf, Cif, ns, m, xr, xi, ring);
             for (int it = 0; it < REFINE_MAX; ++it) {
-                stage_residual(Ms, Ks, ns, nv, ALPH, BETA, h_lu, xr, xi, br, bi, tr, ti, true);
+                stage_residual<true>(Ms, Ks, ns, nv, ALPH, BETA, h_lu, xr, xi, br, bi, tr, ti);
                 double rn = 0.0;
                 for (int r = tid; r < N; r += RNT) rn = fmax(rn, fmax(fabs(tr[r]), fabs(ti[r])));
                 rn = block_max_w<RNW>(rn, S.red);
