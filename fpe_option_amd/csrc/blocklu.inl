@@ -534,7 +534,7 @@ __device__ void schur_pass(GAS T* __restrict__ Pr, GAS T* __restrict__ Pi, long 
         const int r0 = t << 4;
         if (PF && t + RNW < tm) fetch(t + RNW, nr, ni);
         long long oc[4];
-        int dj[4];
+        int dj[4], fi[4];
         bool vc[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -542,11 +542,19 @@ __device__ void schur_pass(GAS T* __restrict__ Pr, GAS T* __restrict__ Pi, long 
             vc[j] = r < M;
             dj[j] = 1;
             oc[j] = vc[j] ? stacked_row(ps, ld, I, m, r, dj[j]) : 0;
+            fi[j] = r - (dj[j] - 1) * m;  // row within block row I + dj
         }
         for (int kb = 0; kb < nkb; ++kb) {
             const T* Br = lds_block<T>(L, CPLX ? 2 * kb : kb);
             const T* Bi = lds_block<T>(L, CPLX ? 2 * kb + 1 : kb);
             const int kk = kb0 + kb;  // K = I + 1 + kk; column block d = kk + 4 - dJ in panel I + dJ
+            // A block's first update (step I = max(J, K) - 3, or 0) finds it as
+            // assembled: zero except within 3 of its diagonal, and only those
+            // entries are written beforehand (the first pass writes the whole
+            // block). Known zeros are not loaded.
+            int band[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) band[j] = (I == 0 || dj[j] == 3 || kk == 2) ? fi[j] : -(1 << 20);
             // TG column tiles of the K block at a time: C tiles loaded up front,
             // independent accumulator chains per tile
             for (int ug = 0; ug < tn; ug += TG) {
@@ -558,7 +566,8 @@ __device__ void schur_pass(GAS T* __restrict__ Pr, GAS T* __restrict__ Pi, long 
                     accr[u] = A4{0, 0, 0, 0}, acci[u] = A4{0, 0, 0, 0};
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
-                        const bool ok = ug + u < tn && vc[j] && c < m;
+                        const bool ok = ug + u < tn && vc[j] && c < m &&
+                                        (band[j] < 0 || (c - band[j] <= 3 && band[j] - c <= 3));
                         const long long o = oc[j] + (long long)(kk + 4 - dj[j]) * m + c;
                         cr[u][j] = ok ? Pr[o] : T(0);
                         ci[u][j] = (CPLX && ok) ? Pi[o] : T(0);
@@ -605,24 +614,45 @@ __device__ void schur_pass(GAS T* __restrict__ Pr, GAS T* __restrict__ Pi, long 
 }
 
 // Write E = a M_s + h K_s (a = ar + i ai) into the panels from the 49-point
-// stencils: a coalesced 16-byte zero fill of the panels, then one thread per
-// row scatters its (at most) 49 stencil entries, whose loads are coalesced
-// across threads (stencil planes are row-contiguous). Panels are 16-byte aligned.
+// stencils, for block_lu: a coalesced zero fill of the blocks it reads before
+// writing them, then one thread per row scatters its (at most) 49 stencil
+// entries, whose loads are coalesced across threads (stencil planes are
+// row-contiguous). Panels are 16-byte aligned.
 template <bool CPLX, class T>
 __device__ void assemble_E(const double* __restrict__ Ms, const double* __restrict__ Ks, int ns, int m,
                            double ar, double ai, double h, T* __restrict__ Pr, T* __restrict__ Pi) {
     const int N = ns * m, ld = 7 * m;
-    const long long tot = (long long)N * ld;
+    // Zeros go only where block_lu reads entries it has not written: block rows
+    // 0..3 whole (row 0 and column 0 are never updated), and the outermost
+    // blocks (d = 0 and 6) of the other rows. Every other in-band block is
+    // updated by the Schur pass, whose first update of a block loads only the
+    // entries within 3 of its diagonal and then writes the whole block.
+    const int head = min(4 * m, N) * ld;  // block rows 0..3, contiguous
     constexpr int V = int(16 / sizeof(T));  // elements per 16-byte store
-    const long long nvec = tot / V;
     const double2 z = {0.0, 0.0};
-    for (long long e = threadIdx.x; e < nvec; e += RNT) {
+    for (int e = threadIdx.x; e < head / V; e += RNT) {
         reinterpret_cast<double2*>(Pr)[e] = z;
         if (CPLX) reinterpret_cast<double2*>(Pi)[e] = z;
     }
-    for (long long e = nvec * V + threadIdx.x; e < tot; e += RNT) {
+    for (int e = head / V * V + threadIdx.x; e < head; e += RNT) {
         Pr[e] = T(0);
         if (CPLX) Pi[e] = T(0);
+    }
+    {
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        for (int row = 4 * m + w; row < N; row += RNT / 64) {
+            const int I = row / m;
+            T* pr = Pr + (long long)row * ld;
+            T* pi = CPLX ? Pi + (long long)row * ld : nullptr;
+            if (lane < m) {
+                pr[lane] = T(0);  // d = 0 (block row I >= 4 > 3)
+                if (CPLX) pi[lane] = T(0);
+                if (I + 3 < ns) {
+                    pr[6 * m + lane] = T(0);
+                    if (CPLX) pi[6 * m + lane] = T(0);
+                }
+            }
+        }
     }
     __syncthreads();
     for (int row = threadIdx.x; row < N; row += RNT) {
