@@ -615,9 +615,8 @@ __device__ void schur_pass(GAS T* __restrict__ Pr, GAS T* __restrict__ Pi, long 
 
 // Write E = a M_s + h K_s (a = ar + i ai) into the panels from the 49-point
 // stencils, for block_lu: a coalesced zero fill of the blocks it reads before
-// writing them, then one thread per row scatters its (at most) 49 stencil
-// entries, whose loads are coalesced across threads (stencil planes are
-// row-contiguous). Panels are 16-byte aligned.
+// writing them, then the (at most) 49 stencil entries of every row (stencil
+// planes are row-contiguous). Panels are 16-byte aligned.
 template <bool CPLX, class T>
 __device__ void assemble_E(const double* __restrict__ Ms, const double* __restrict__ Ks, int ns, int m,
                            double ar, double ai, double h, T* __restrict__ Pr, T* __restrict__ Pi) {
@@ -655,26 +654,45 @@ __device__ void assemble_E(const double* __restrict__ Ms, const double* __restri
         }
     }
     __syncthreads();
-    for (int row = threadIdx.x; row < N; row += RNT) {
-        const int I = row / m, i = row - I * m;
-        T* pr = Pr + (long long)row * ld;
-        T* pi = CPLX ? Pi + (long long)row * ld : nullptr;
-        for (int d = 0; d < 7; ++d) {
-            const int J = I + d - 3;
-            if (J < 0 || J >= ns) continue;
-            double mv[7], kv[7];
+    // Stencil entries: a wave takes 8 rows at a time, lane = (row ri, entry q)
+    // for 56 of its lanes, so a load reads 8 consecutive rows of 7 planes and a
+    // store writes 8 runs of 7 consecutive entries. Two such groups go together:
+    // all 28 loads are issued before the stores.
+    {
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        const int ri = lane / 7, q = lane - 7 * (lane / 7);
+        constexpr int G = 8 * (RNT / 64);  // rows per pass of all waves
+        for (int r0 = 8 * w; r0 < N; r0 += 2 * G) {
+            int rc[2];
+            bool act[2];
+            double mv[2][7], kv[2][7];
 #pragma unroll
-            for (int q = 0; q < 7; ++q) {  // every plane has an entry per row (zero for missing neighbours)
-                const long long o = (long long)(d * 7 + q) * N + row;
-                mv[q] = Ms[o];
-                kv[q] = Ks[o];
+            for (int g = 0; g < 2; ++g) {
+                const int row = r0 + g * G + ri;
+                act[g] = lane < 56 && row < N;
+                rc[g] = act[g] ? row : 0;  // every plane has an entry per row (zero for missing neighbours)
+#pragma unroll
+                for (int d = 0; d < 7; ++d) {
+                    const long long o = (long long)(d * 7 + q) * N + rc[g];
+                    mv[g][d] = Ms[o];
+                    kv[g][d] = Ks[o];
+                }
             }
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int q = 0; q < 7; ++q) {
-                const int j = i + q - 3;
-                if (j >= 0 && j < m) {
-                    pr[d * m + j] = T(ar * mv[q] + h * kv[q]);
-                    if (CPLX) pi[d * m + j] = T(ai * mv[q]);
+            for (int g = 0; g < 2; ++g) {
+                const int I = rc[g] / m, i = rc[g] - I * m, j = i + q - 3;
+                if (act[g] && j >= 0 && j < m) {
+                    T* pr = Pr + (long long)rc[g] * ld + j;
+                    T* pi = CPLX ? Pi + (long long)rc[g] * ld + j : nullptr;
+#pragma unroll
+                    for (int d = 0; d < 7; ++d) {
+                        const int J = I + d - 3;
+                        if (J >= 0 && J < ns) {
+                            pr[d * m] = T(ar * mv[g][d] + h * kv[g][d]);
+                            if (CPLX) pi[d * m] = T(ai * mv[g][d]);
+                        }
+                    }
                 }
             }
         }
