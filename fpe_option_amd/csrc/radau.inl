@@ -465,24 +465,39 @@ __global__ void __launch_bounds__(RNT, RADAU_WG_PER_CU * RNW / 4) k_radau(const 
                             lap(tk_mv);
                             block_solve_multi<2>(P, ns, m, vv(S_WRE), vv(S_WIM), nullptr, ring);
                             lap(tk_solve);
-                            for (int k = 0; k <= j; ++k) {
-                                double hv = 0.0;
-                                for (int r = tid; r < N; r += RNT)
-                                    hv += vv(S_WRE)[r] * vv(S_VRE0 + k)[r] + vv(S_WIM)[r] * vv(S_VIM0 + k)[r];
-                                hv = block_sum_w<RNW>(hv, S.red);
-                                if (tid == 0) S.H[k * GM_K + j] = hv;
-                                for (int r = tid; r < N; r += RNT) {
-                                    vv(S_WRE)[r] -= hv * vv(S_VRE0 + k)[r];
-                                    vv(S_WIM)[r] -= hv * vv(S_VIM0 + k)[r];
-                                }
-                                __syncthreads();
-                            }
+                            // modified Gram-Schmidt; each thread owns its entries r, so the
+                            // update by V_k and the next projection (or the final norm) share
+                            // one pass with the same per-entry arithmetic as two passes
+                            double hv = 0.0;
+                            for (int r = tid; r < N; r += RNT)
+                                hv += vv(S_WRE)[r] * vv(S_VRE0)[r] + vv(S_WIM)[r] * vv(S_VIM0)[r];
+                            hv = block_sum_w<RNW>(hv, S.red);
                             double wn = 0.0;
-                            for (int r = tid; r < N; r += RNT) {
-                                const double a = vv(S_WRE)[r], b = vv(S_WIM)[r];
-                                wn += a * a + b * b;
+                            for (int k = 0; k <= j; ++k) {
+                                if (tid == 0) S.H[k * GM_K + j] = hv;
+                                const double* vr = vv(S_VRE0 + k);
+                                const double* vi = vv(S_VIM0 + k);
+                                double* wr = vv(S_WRE);
+                                double* wi = vv(S_WIM);
+                                double nx = 0.0;
+                                if (k < j) {
+                                    const double* ur = vv(S_VRE0 + k + 1);
+                                    const double* ui = vv(S_VIM0 + k + 1);
+                                    for (int r = tid; r < N; r += RNT) {
+                                        const double a = wr[r] - hv * vr[r], b = wi[r] - hv * vi[r];
+                                        wr[r] = a, wi[r] = b;
+                                        nx += a * ur[r] + b * ui[r];
+                                    }
+                                    hv = block_sum_w<RNW>(nx, S.red);
+                                } else {
+                                    for (int r = tid; r < N; r += RNT) {
+                                        const double a = wr[r] - hv * vr[r], b = wi[r] - hv * vi[r];
+                                        wr[r] = a, wi[r] = b;
+                                        nx += a * a + b * b;
+                                    }
+                                    wn = sqrt(block_sum_w<RNW>(nx, S.red));
+                                }
                             }
-                            wn = sqrt(block_sum_w<RNW>(wn, S.red));
                             if (wn > 1e-30) {
                                 const double iw = 1.0 / wn;
                                 for (int r = tid; r < N; r += RNT) {
