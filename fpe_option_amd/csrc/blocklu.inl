@@ -704,12 +704,14 @@ __device__ void assemble_E(const double* __restrict__ Ms, const double* __restri
 // so each load instruction reads one contiguous row segment, and the loads of
 // half the rows are issued before their stores (two memory latencies per
 // block). Padding columns of the staged rows are written with zeros.
-template <bool CPLX, class T>
+// H = 1 issues the loads of all rows before any store (one memory latency,
+// twice the registers).
+template <bool CPLX, class T, int H = 2>
 __device__ __forceinline__ void stage_in(T* dr, T* di, GAS const T* sr, GAS const T* si, int ld, int m) {
-    constexpr int R = MAX_BLOCK / RNW / 2;
+    constexpr int R = MAX_BLOCK / RNW / H;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < H; ++h) {
         T vr[R], vi[R];
 #pragma unroll
         for (int u = 0; u < R; ++u) vr[u] = vi[u] = T(0);
@@ -760,12 +762,13 @@ __device__ __forceinline__ void stage_out(const T* dr, const T* di, GAS T* sr, G
     }
 }
 // The pivot block's copies in and out of LDS as calls of their own, so that
-// their registers do not add to the factorisation loop's.
+// their registers do not add to the factorisation loop's (the copy in issues
+// every row's loads at once).
 template <bool CPLX, class T>
 __device__ __noinline__ void pivot_block_in(LuLds& L, const T* sr, const T* si, int ld, int m) {
     m = uniform(m), ld = uniform(ld);
-    stage_in<CPLX, T>(lds_block<T>(L, 0), lds_block<T>(L, 1), uniform(gas(sr)), CPLX ? uniform(gas(si)) : nullptr,
-                      ld, m);
+    stage_in<CPLX, T, 1>(lds_block<T>(L, 0), lds_block<T>(L, 1), uniform(gas(sr)), CPLX ? uniform(gas(si)) : nullptr,
+                         ld, m);
 }
 template <bool CPLX, class T>
 __device__ __noinline__ void pivot_block_out(LuLds& L, T* sr, T* si, int ld, int m) {
@@ -816,7 +819,7 @@ __device__ __noinline__ int block_lu(T* __restrict__ Pr_, T* __restrict__ Pi_, i
                 T* Br = lds_block<T>(L, CPLX ? 2 * kb : kb);
                 T* Bi = CPLX ? lds_block<T>(L, 2 * kb + 1) : nullptr;
                 const int col = (4 + kb0 + kb) * m;
-                stage_in<CPLX, T>(Br, Bi, PIr + col, CPLX ? PIi + col : nullptr, ld, m);
+                stage_in<CPLX, T, 1>(Br, Bi, PIr + col, CPLX ? PIi + col : nullptr, ld, m);
             }
             __syncthreads();
             lap(1);
