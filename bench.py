@@ -136,6 +136,9 @@ def mixed_leg(api, cols, prices, status, plain_value, n, precision, path):
                                                                           mst["n_err_solves"]), 3),
             "newton_iterations": int(mst["n_newton"]), "lu_refreshes": int(mst["n_lu"]),
             "phases_ms": {k: round(mst[k], 1) for k in ("ms_ic", "ms_radau")},
+            "radau_wg_s": {k[5:]: round(mst[k], 1) for k in (
+                "wg_s_assemble", "wg_s_lu", "wg_s_solve", "wg_s_spmv", "wg_s_other", "wg_s_total",
+                "wg_s_lu_inverse", "wg_s_lu_stage", "wg_s_lu_lmul", "wg_s_lu_schur")},
             "status_ok": int(np.sum(mstatus == 0))}
 
 
