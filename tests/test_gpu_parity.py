@@ -149,6 +149,33 @@ def test_mixed_batch_against_oracle(engine, oracle):
     assert n_exact >= n - 2, f"only {n_exact}/{n} options within 1e-8"
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_tiny_grids_against_oracle(engine, oracle, precision):
+    """Grids of 4..7 dofs per axis: fewer than four block rows (the whole band is
+    zero-filled before assembly) and narrow, odd blocks."""
+    rng = np.random.default_rng(7)
+    grids = [(4, 4), (5, 7), (6, 5), (7, 6), (5, 4), (7, 7)]
+    n = len(grids)
+    types = np.array([8, 9, 2, 1, 6, 4])
+    cols = dict(kappa=rng.uniform(0.5, 4, n), theta=rng.uniform(0.02, 0.2, n), sigma=rng.uniform(0.1, 0.8, n),
+                rho=rng.uniform(-0.9, 0, n), V0=rng.uniform(0.02, 0.3, n), r=rng.uniform(0, 0.1, n),
+                T=rng.uniform(0.1, 2, n), S0=60.0, strike=rng.uniform(45, 75, n), option_type=types,
+                disc_mode=1, n_s=np.array([g[0] for g in grids]), n_v=np.array([g[1] for g in grids]), v_max=1.5,
+                s_max=180.0, ic_sigma_s=0.5, ic_sigma_v=0.005, rtol=1e-4, atol=1e-6, complex_mode=1,
+                precision=precision)
+    u = rng.uniform(0.1, 0.25, n)
+    cols["barrier"] = np.where(types >= 8, 0.0, np.where(types <= 3, 60.0 * (1 - u), 60.0 * (1 + u)))
+    from fpe_option_amd import api
+    g, status = api.price_batch(**cols)
+    assert np.all(status == 0)
+    for i in range(n):
+        kw = {k: (v[i].item() if isinstance(v, np.ndarray) else v) for k, v in cols.items()
+              if k not in ("strike", "precision")}
+        o, _, rc = oracle.price([cols["strike"][i]], **kw)
+        assert rc == 0
+        _close([g[i]], o, rtol=NEWTON_BOUND, atol=1e-12)
+
+
 def test_stepwise_compute_matches_oracle(engine, oracle):
     import fpe_engine as fe
     c = fe.Compute(**HESTON, n_s=16, n_v=16, option_type="down_and_out_call", barrier=50.0, num_insert=50)
