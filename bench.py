@@ -174,7 +174,7 @@ def main():
         copy_calibration(args.calibrate_copy)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU sharding is design-only",
+        print(json.dumps({"error": "this benchmark runs on one GPU; sharding over several GPUs is design-only",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 

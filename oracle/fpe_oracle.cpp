@@ -1,6 +1,6 @@
 // FP64 CPU oracle for the Heston Fokker-Planck pricing path.
 //
-// TEST INFRASTRUCTURE ONLY: the checker the GPU path is compared against.
+// TEST INFRASTRUCTURE ONLY: what the GPU path is compared against.
 // Nothing on the product path (fpe_option_amd/, include/) links or calls it.
 // Reference file:line citations are relative to the reference repository root.
 #include "fpe_oracle.h"

@@ -1,7 +1,7 @@
 /* FP64 CPU oracle for the Heston Fokker-Planck pricing path.
  *
- * TEST INFRASTRUCTURE ONLY. This library is the checker that the GPU path is
- * compared against. Only tests/, __graft_entry__.smoke() and bench.py's CPU
+ * TEST INFRASTRUCTURE ONLY. This library is what the GPU path is compared
+ * against. Only tests/, __graft_entry__.smoke() and bench.py's CPU
  * baseline leg may load it; nothing on the product path links or calls it.
  *
  * It restates, single-threaded and in plain FP64, the reference engine's

@@ -1,6 +1,7 @@
 """ctypes binding of the FP64 CPU oracle (oracle/fpe_oracle.h).
 
-Test infrastructure only: the oracle is the checker, never the thing measured.
+Test infrastructure only: results are compared against the oracle; it is never
+the thing measured.
 """
 import ctypes as C
 import os

@@ -1,4 +1,4 @@
-"""Pins the FP64 CPU oracle before it is trusted as the GPU's checker.
+"""Pins the FP64 CPU oracle before the GPU path is compared against it.
 
 Each check is chosen so that a plausible restatement mistake fails it:
 reference-held printed prices (README.md:385-403,661-683 and
