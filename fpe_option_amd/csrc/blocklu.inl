@@ -984,9 +984,14 @@ __device__ __noinline__ void block_solve1(const T* __restrict__ Pr_, const T* __
 // entries are loaded once, one block ahead, and contracted with every
 // right-hand side; each right-hand side keeps its last solution blocks in its
 // own LDS ring. ring: LDS of at least 5 * NR * 64 doubles.
-template <int NR>
-__device__ __noinline__ void block_solve_multi(const double* __restrict__ P_, int ns, int m, double* x0_,
-                                               double* x1_, double* x2_, double* ring) {
+// NF > 0: a block width m with m / 4 == NF known at compile time, so the NF
+// full column groups (every lane's column inside the block) are loaded and
+// contracted without per-lane predicates, and only group NF (m % 4 columns)
+// is predicated. Rows i >= m then read row m - 1 and compute values nobody
+// stores. NF = 0: any m, every entry predicated.
+template <int NR, int NF>
+__device__ __noinline__ void block_solve_multi_w(const double* __restrict__ P_, int ns, int m, double* x0_,
+                                                 double* x1_, double* x2_, double* ring) {
     static_assert(NR >= 1 && NR <= 3, "one to three right-hand sides");
     GAS const double* const P = uniform(gas(P_));
     GAS double* x[3] = {uniform(gas(x0_)), NR > 1 ? uniform(gas(x1_)) : nullptr,
@@ -1000,31 +1005,42 @@ __device__ __noinline__ void block_solve_multi(const double* __restrict__ P_, in
     const bool row = i < m;
     double* rg = ring;                // ring of right-hand side v: rg + v * 256, slot J & 3
     double* tv = ring + NR * 4 * 64;  // temporaries: tv + v * 64
-    double a0[U], a1[U];
+    constexpr int UA = NF > 0 ? NF + 1 : U;  // column groups held per block
+    const int ic = NF > 0 ? min(i, m - 1) : i;
+    double a0[UA], a1[UA];
     GAS const double* pr = nullptr;
-    auto load = [&](int col, bool ok, double* a) {
+    auto fill = [&](GAS const double* qr, bool ok, double* a) {
+        if constexpr (NF > 0) {
+            if (ok) {
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int k = q + TPR * u;
-            a[u] = (ok && row && k < m) ? pr[col + k] : 0.0;
+                for (int u = 0; u < NF; ++u) a[u] = qr[q + TPR * u];
+                const int k = q + TPR * NF;
+                a[NF] = k < m ? qr[k] : 0.0;
+            } else {
+#pragma unroll
+                for (int u = 0; u < UA; ++u) a[u] = 0.0;
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = q + TPR * u;
+                a[u] = (ok && row && k < m) ? qr[k] : 0.0;
+            }
         }
     };
-    auto load_row = [&](int I, int col, bool ok, double* a) {
-        GAS const double* qr = P + I * ps + (long long)i * ld;
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int k = q + TPR * u;
-            a[u] = (ok && row && k < m) ? qr[col + k] : 0.0;
-        }
-    };
+    auto load = [&](int col, bool ok, double* a) { fill(pr + col, ok, a); };
+    auto load_row = [&](int I, int col, bool ok, double* a) { fill(P + I * ps + (long long)ic * ld + col, ok, a); };
     auto contract = [&](const double* a, int slot, double* acc) {
 #pragma unroll
         for (int v = 0; v < NR; ++v) {
             const double* y = rg + v * 256 + slot * 64;
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
+            for (int u = 0; u < UA; ++u) {
                 const int k = q + TPR * u;
-                if (k < m) acc[v] += a[u] * y[k];
+                if (NF > 0 && u < NF)
+                    acc[v] += a[u] * y[k];
+                else if (k < m)
+                    acc[v] += a[u] * y[k];
             }
         }
     };
@@ -1044,7 +1060,7 @@ __device__ __noinline__ void block_solve_multi(const double* __restrict__ P_, in
     auto d0_of = [&](int I) { return I >= 3 ? 0 : 3 - I; };
     if (ns > 0) load_row(0, 0, d0_of(0) <= 0, a0);
     for (int I = 0; I < ns; ++I) {
-        pr = P + I * ps + (long long)i * ld;
+        pr = P + I * ps + (long long)ic * ld;
         const int d0 = d0_of(I);
         double acc[NR];
 #pragma unroll
@@ -1056,7 +1072,7 @@ __device__ __noinline__ void block_solve_multi(const double* __restrict__ P_, in
         if (I + 1 < ns) load_row(I + 1, 0, d0_of(I + 1) <= 0, a1);
         contract(a0, (I + 3) & 3, acc);
 #pragma unroll
-        for (int u = 0; u < U; ++u) a0[u] = a1[u];
+        for (int u = 0; u < UA; ++u) a0[u] = a1[u];
         reduce4(acc);
         if (row && q == 0) {
 #pragma unroll
@@ -1073,7 +1089,7 @@ __device__ __noinline__ void block_solve_multi(const double* __restrict__ P_, in
     auto nK_of = [&](int I) { return (ns - 1 - I) < 3 ? (ns - 1 - I) : 3; };
     if (ns > 0) load_row(ns - 1, 4 * m, nK_of(ns - 1) > 0, a0);
     for (int I = ns - 1; I >= 0; --I) {
-        pr = P + I * ps + (long long)i * ld;
+        pr = P + I * ps + (long long)ic * ld;
         const int nK = nK_of(I);
         double acc[NR];
 #pragma unroll
@@ -1095,9 +1111,12 @@ __device__ __noinline__ void block_solve_multi(const double* __restrict__ P_, in
         for (int v = 0; v < NR; ++v) {
             acc[v] = 0.0;
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
+            for (int u = 0; u < UA; ++u) {
                 const int k = q + TPR * u;
-                if (k < m) acc[v] += a1[u] * tv[v * 64 + k];
+                if (NF > 0 && u < NF)
+                    acc[v] += a1[u] * tv[v * 64 + k];
+                else if (k < m)
+                    acc[v] += a1[u] * tv[v * 64 + k];
             }
         }
         reduce4(acc);
@@ -1112,5 +1131,18 @@ __device__ __noinline__ void block_solve_multi(const double* __restrict__ P_, in
     }
 }
 
+
+// Block solve with NR right-hand sides, compiled for the common block widths.
+template <int NR>
+__device__ __forceinline__ void block_solve_multi(const double* P, int ns, int m, double* x0, double* x1, double* x2,
+                                                  double* ring) {
+    switch (m >> 2) {
+        case 5: block_solve_multi_w<NR, 5>(P, ns, m, x0, x1, x2, ring); break;
+        case 7: block_solve_multi_w<NR, 7>(P, ns, m, x0, x1, x2, ring); break;
+        case 10: block_solve_multi_w<NR, 10>(P, ns, m, x0, x1, x2, ring); break;
+        case 15: block_solve_multi_w<NR, 15>(P, ns, m, x0, x1, x2, ring); break;
+        default: block_solve_multi_w<NR, 0>(P, ns, m, x0, x1, x2, ring); break;
+    }
+}
 
 }  // namespace fpe
