@@ -840,9 +840,11 @@ __device__ __noinline__ int block_lu(T* __restrict__ Pr_, T* __restrict__ Pi_, i
 // ring (4 x 64, re and im) instead of being re-read from memory. ring: LDS of
 // at least 10 * 64 doubles. Factors of type T (FP64, or FP32 for the
 // mixed-precision path) are widened to FP64; all arithmetic is FP64.
-template <bool CPLX, class T>
-__device__ __noinline__ void block_solve1(const T* __restrict__ Pr_, const T* __restrict__ Pi_, int ns, int m,
-                                          double* __restrict__ xr_, double* __restrict__ xi_, double* ring) {
+// NF as in block_solve_multi_w: m / 4 known at compile time (NF > 0) lets the
+// full column groups go unpredicated; NF = 0 takes any m.
+template <bool CPLX, class T, int NF>
+__device__ __noinline__ void block_solve1_w(const T* __restrict__ Pr_, const T* __restrict__ Pi_, int ns, int m,
+                                            double* __restrict__ xr_, double* __restrict__ xi_, double* ring) {
     GAS const T* const Pr = uniform(gas(Pr_));
     GAS const T* const Pi = uniform(gas(Pi_));
     GAS double* const xr = uniform(gas(xr_));
@@ -858,38 +860,50 @@ __device__ __noinline__ void block_solve1(const T* __restrict__ Pr_, const T* __
     double* ri = ring + 4 * 64;
     double* tr = ring + 8 * 64;
     double* ti = ring + 9 * 64;
-    double a0[U], b0[U], a1[U], b1[U];
+    constexpr int UA = NF > 0 ? NF + 1 : U;  // column groups held per block
+    const int ic = NF > 0 ? min(i, m - 1) : i;  // NF > 0: rows past m read row m - 1, unstored
+    double a0[UA], b0[UA], a1[UA], b1[UA];
     GAS const T* pr = nullptr;
     GAS const T* pi = nullptr;
+    auto fill = [&](GAS const T* qr, GAS const T* qi, bool ok, double* a, double* b) {
+        if constexpr (NF > 0) {
+            if (ok) {
+#pragma unroll
+                for (int u = 0; u < NF; ++u) {
+                    a[u] = double(qr[q + TPR * u]);
+                    b[u] = CPLX ? double(qi[q + TPR * u]) : 0.0;
+                }
+                const int k = q + TPR * NF;
+                a[NF] = k < m ? double(qr[k]) : 0.0;
+                b[NF] = (CPLX && k < m) ? double(qi[k]) : 0.0;
+            } else {
+#pragma unroll
+                for (int u = 0; u < UA; ++u) a[u] = 0.0, b[u] = 0.0;
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = q + TPR * u;
+                const bool v = ok && row && k < m;
+                a[u] = v ? double(qr[k]) : 0.0;
+                b[u] = (CPLX && v) ? double(qi[k]) : 0.0;
+            }
+        }
+    };
     // factor entries (row i, columns col + q + 4u) of block row I
     auto load_row = [&](int I, int col, bool ok, double* a, double* b) {
-        GAS const T* qr = Pr + I * ps + (long long)i * ld;
-        GAS const T* qi = CPLX ? Pi + I * ps + (long long)i * ld : nullptr;
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int k = q + TPR * u;
-            const bool v = ok && row && k < m;
-            a[u] = v ? double(qr[col + k]) : 0.0;
-            b[u] = (CPLX && v) ? double(qi[col + k]) : 0.0;
-        }
+        fill(Pr + I * ps + (long long)ic * ld + col, CPLX ? Pi + I * ps + (long long)ic * ld + col : nullptr, ok, a,
+             b);
     };
     // ... of the current block row
-    auto load = [&](int col, bool ok, double* a, double* b) {
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int k = q + TPR * u;
-            const bool v = ok && row && k < m;
-            a[u] = v ? double(pr[col + k]) : 0.0;
-            b[u] = (CPLX && v) ? double(pi[col + k]) : 0.0;
-        }
-    };
+    auto load = [&](int col, bool ok, double* a, double* b) { fill(pr + col, CPLX ? pi + col : nullptr, ok, a, b); };
     // (ar, ai) += block row segment (a, b) times the vector block (yr, yi)
     auto contract = [&](const double* a, const double* b, const double* yr, const double* yi, double& ar,
                         double& ai) {
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
+        for (int u = 0; u < UA; ++u) {
             const int k = q + TPR * u;
-            if (k < m) {
+            if ((NF > 0 && u < NF) || k < m) {
                 if (CPLX) {
                     ar += a[u] * yr[k] - b[u] * yi[k];
                     ai += a[u] * yi[k] + b[u] * yr[k];
@@ -917,8 +931,8 @@ __device__ __noinline__ void block_solve1(const T* __restrict__ Pr_, const T* __
     auto d0_of = [&](int I) { return I >= 3 ? 0 : 3 - I; };  // blocks d < d0 lie left of the matrix
     if (ns > 0) load_row(0, 0, d0_of(0) <= 0, a0, b0);
     for (int I = 0; I < ns; ++I) {
-        pr = Pr + I * ps + (long long)i * ld;
-        pi = CPLX ? Pi + I * ps + (long long)i * ld : nullptr;
+        pr = Pr + I * ps + (long long)ic * ld;
+        pi = CPLX ? Pi + I * ps + (long long)ic * ld : nullptr;
         const int d0 = d0_of(I);
         double ar = 0.0, ai = 0.0;
         load(m, d0 <= 1, a1, b1);
@@ -928,7 +942,7 @@ __device__ __noinline__ void block_solve1(const T* __restrict__ Pr_, const T* __
         if (I + 1 < ns) load_row(I + 1, 0, d0_of(I + 1) <= 0, a1, b1);
         contract(a0, b0, rr + ((I + 3) & 3) * 64, ri + ((I + 3) & 3) * 64, ar, ai);
 #pragma unroll
-        for (int u = 0; u < U; ++u) a0[u] = a1[u], b0[u] = b1[u];
+        for (int u = 0; u < UA; ++u) a0[u] = a1[u], b0[u] = b1[u];
         reduce4(ar, ai);
         if (row && q == 0) {
             const double vr = xr[I * m + i] - ar;
@@ -947,8 +961,8 @@ __device__ __noinline__ void block_solve1(const T* __restrict__ Pr_, const T* __
     auto nK_of = [&](int I) { return (ns - 1 - I) < 3 ? (ns - 1 - I) : 3; };  // blocks right of the matrix are skipped
     if (ns > 0) load_row(ns - 1, 4 * m, nK_of(ns - 1) > 0, a0, b0);
     for (int I = ns - 1; I >= 0; --I) {
-        pr = Pr + I * ps + (long long)i * ld;
-        pi = CPLX ? Pi + I * ps + (long long)i * ld : nullptr;
+        pr = Pr + I * ps + (long long)ic * ld;
+        pi = CPLX ? Pi + I * ps + (long long)ic * ld : nullptr;
         const int nK = nK_of(I);
         double ar = 0.0, ai = 0.0;
         load(5 * m, nK > 1, a1, b1);
@@ -1131,6 +1145,19 @@ __device__ __noinline__ void block_solve_multi_w(const double* __restrict__ P_, 
     }
 }
 
+
+// Single right-hand side block solve, compiled for the common block widths.
+template <bool CPLX, class T>
+__device__ __forceinline__ void block_solve1(const T* Pr, const T* Pi, int ns, int m, double* xr, double* xi,
+                                             double* ring) {
+    switch (m >> 2) {
+        case 5: block_solve1_w<CPLX, T, 5>(Pr, Pi, ns, m, xr, xi, ring); break;
+        case 7: block_solve1_w<CPLX, T, 7>(Pr, Pi, ns, m, xr, xi, ring); break;
+        case 10: block_solve1_w<CPLX, T, 10>(Pr, Pi, ns, m, xr, xi, ring); break;
+        case 15: block_solve1_w<CPLX, T, 15>(Pr, Pi, ns, m, xr, xi, ring); break;
+        default: block_solve1_w<CPLX, T, 0>(Pr, Pi, ns, m, xr, xi, ring); break;
+    }
+}
 
 // Block solve with NR right-hand sides, compiled for the common block widths.
 template <int NR>
