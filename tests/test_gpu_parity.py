@@ -176,6 +176,26 @@ def test_tiny_grids_against_oracle(engine, oracle, precision):
         _close([g[i]], o, rtol=NEWTON_BOUND, atol=1e-12)
 
 
+@pytest.mark.parametrize("complex_mode,precision", [(1, 0), (0, 0), (1, 1)])
+def test_block_widths_against_oracle(engine, oracle, complex_mode, precision):
+    """The block widths the solves are compiled for (30 and 40 here; 20 in the
+    config tests), in direct, GMRES and mixed-precision modes."""
+    from fpe_option_amd import api
+    grids = [(60, 30), (80, 40)]
+    base = dict(HESTON, disc_mode=1, v_max=1.5, s_max=180.0, ic_sigma_s=0.5, ic_sigma_v=0.005, rtol=1e-4,
+                atol=1e-6, option_type=8, barrier=0.0)
+    g, status = api.price_batch(strike=60.0, n_s=[a for a, _ in grids], n_v=[b for _, b in grids],
+                                complex_mode=complex_mode, precision=precision, **base)
+    assert np.all(status == 0)
+    for k, (ns, nv) in enumerate(grids):
+        o, _, rc = oracle.price([60.0], **base, n_s=ns, n_v=nv, complex_mode=complex_mode)
+        assert rc == 0
+        if complex_mode == 1:
+            _close([g[k]], o, rtol=NEWTON_BOUND, atol=1e-12)
+        else:  # the reference's GMRES stops anywhere inside its Newton tolerance
+            _close([g[k]], o, rtol=1e-5, atol=GMRES_ABS)
+
+
 def test_stepwise_compute_matches_oracle(engine, oracle):
     import fpe_engine as fe
     c = fe.Compute(**HESTON, n_s=16, n_v=16, option_type="down_and_out_call", barrier=50.0, num_insert=50)
