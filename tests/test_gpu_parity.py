@@ -196,6 +196,33 @@ def test_block_widths_against_oracle(engine, oracle, complex_mode, precision):
             _close([g[k]], o, rtol=1e-5, atol=GMRES_ABS)
 
 
+def test_config2_size_properties(engine, oracle):
+    """The single-instance 120 x 60 configuration (7200 dofs, rtol 1e-8), checked
+    through properties that hold at any size: the Heston (1993) closed form to
+    discretisation accuracy, put-call parity, barrier ordering and in-out
+    parity, and agreement of the mixed-precision path with the direct FP64
+    one. All options run in one batch (GMRES mode at this size and tolerance
+    takes minutes; the smaller tests cover it)."""
+    from fpe_option_amd import api
+    r, T, S0, K = HESTON["r"], HESTON["T"], HESTON["S0"], 60.0
+    # call, put, DOC, DIC (direct FP64); call in mixed precision
+    types = [8, 9, 2, 0, 8]
+    cmode = [1, 1, 1, 1, 1]
+    prec = [0, 0, 0, 0, 1]
+    barrier = [0.0, 0.0, 50.0, 50.0, 0.0]
+    g, status = api.price_batch(**HESTON, strike=K, option_type=types, barrier=barrier, complex_mode=cmode,
+                                precision=prec, disc_mode=1, n_s=120, n_v=60, v_max=1.5, s_max=180.0,
+                                ic_sigma_s=0.5, ic_sigma_v=0.005, rtol=1e-8, atol=1e-11)
+    assert np.all(status == 0)
+    call, put, doc, dic, call_mixed = g
+    ref = oracle.lib().orc_heston_call(1.2, 0.05, 0.35, -0.4, r, T, S0, 0.1, K)
+    assert abs(call / ref - 1.0) < 5e-3
+    assert abs((call - put) - (S0 - K * np.exp(-r * T))) < 1e-2
+    assert 0.0 < doc < call and dic > 0.0
+    assert abs(doc + dic - call) < 1e-12 * call  # in-out parity (the in-option is priced by it)
+    assert abs(call_mixed - call) < 1e-8
+
+
 def test_stepwise_compute_matches_oracle(engine, oracle):
     import fpe_engine as fe
     c = fe.Compute(**HESTON, n_s=16, n_v=16, option_type="down_and_out_call", barrier=50.0, num_insert=50)
