@@ -14,7 +14,8 @@ LIB_PATH = os.path.join(HERE, "libfpe_engine.so")
 DISC_REFERENCE, DISC_SPAN = 0, 1
 CPLX_GMRES, CPLX_DIRECT = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
-STATUS_NAMES = {0: "OK", 1: "INVALID", 2: "MAX_STEPS", 3: "H_UNDERFLOW", 4: "NONFINITE", 5: "TOO_LARGE"}
+STATUS_NAMES = {0: "OK", 1: "INVALID", 2: "MAX_STEPS", 3: "H_UNDERFLOW", 4: "NONFINITE", 5: "TOO_LARGE",
+                6: "SINGULAR", 7: "REFINE_FAIL"}
 
 
 class FpeVecResult(C.Structure):
@@ -73,6 +74,15 @@ class FpeBatchStats(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class FpeSolveInfo(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("status", "n_solves", "n_accepted", "n_rejected", "n_lu", "n_newton",
+                                          "n_err_solves", "n_inner_iters", "n_admm_iters", "n_refine", "n_sb",
+                                          "n_vb")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 # Every symbol the header declares, with its ctypes signature.
 _D, _I, _P = C.c_double, C.c_int32, C.c_void_p
 _DP = C.POINTER(C.c_double)
@@ -97,6 +107,10 @@ SIGNATURES = {
     "fpe_compute_free_csr": ([C.POINTER(FpeCsrResult)], None),
     "fpe_batch_option_default": ([C.POINTER(FpeBatchOption)], None),
     "fpe_batch_price": ([C.POINTER(FpeBatchOption), _I, _DP, C.POINTER(C.c_int32), C.POINTER(FpeBatchStats)], _I),
+    "fpe_batch_price_detail": ([C.POINTER(FpeBatchOption), _I, _DP, C.POINTER(C.c_int32), C.POINTER(FpeSolveInfo),
+                                _DP, C.c_int64, C.POINTER(FpeBatchStats)], _I),
+    "fpe_batch_operators": ([C.POINTER(FpeBatchOption), _I, _DP, _DP, C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
+                            _I),
     "fpe_batch_terminal_states": ([C.POINTER(FpeBatchOption), _I, _DP, C.c_int64, C.POINTER(C.c_int32),
                                    C.POINTER(C.c_int32), C.POINTER(FpeBatchStats)], _I),
     "fpe_european_pricing_operator": ([C.POINTER(FpeBatchOption), _I, _I, _DP, _DP, C.POINTER(C.c_int32),
@@ -152,13 +166,30 @@ def batch_options(n):
     return arr
 
 
-def price_batch_raw(arr):
+MAX_DOF = 300 * 64  # largest grid the kernels take (n_sb <= 300, n_vb <= 64)
+
+
+def price_batch_raw(arr, want_info=False, want_qT=False):
+    """fpe_batch_price_detail over an FpeBatchOption array: prices, status,
+    batch statistics, then the per-option records (want_info) and the list of
+    terminal states (want_qT; None for in-options) when asked."""
     n = len(arr)
     prices = np.zeros(n)
     status = np.zeros(n, dtype=np.int32)
     st = FpeBatchStats()
-    rc = lib().fpe_batch_price(arr, n, prices.ctypes.data_as(_DP),
-                               status.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(st))
+    info = (FpeSolveInfo * max(n, 1))()
+    q = np.zeros((n, MAX_DOF)) if want_qT else None
+    rc = lib().fpe_batch_price_detail(arr, n, prices.ctypes.data_as(_DP),
+                                      status.ctypes.data_as(C.POINTER(C.c_int32)), info,
+                                      q.ctypes.data_as(_DP) if want_qT else None, MAX_DOF if want_qT else 0,
+                                      C.byref(st))
     if rc != 0:
         raise RuntimeError(f"fpe_batch_price failed ({rc}): {last_error()}")
-    return prices, status, st.as_dict()
+    out = (prices, status, st.as_dict())
+    recs = [info[i].as_dict() for i in range(n)]
+    if want_info:
+        out += (recs,)
+    if want_qT:
+        out += ([q[i, :r["n_sb"] * r["n_vb"]].copy() if r["n_solves"] == 1 and r["status"] in (0, 7) else None
+                 for i, r in enumerate(recs)],)
+    return out

@@ -332,10 +332,7 @@ _BATCH_FIELDS = ("kappa", "theta", "sigma", "rho", "r", "T", "S0", "V0", "s_min"
                  "knot_center_v")
 
 
-def price_batch(return_stats: bool = False, **columns):
-    """Price many independent options in one GPU call (extension; no
-    counterpart in the reference). Each keyword is a scalar or an array of one
-    value per option; names follow FpeBatchOption. Returns (prices, status)."""
+def _batch_array(columns):
     n = max([np.size(v) for v in columns.values()] + [1])
     arr = L.batch_options(n)
     for k, v in columns.items():
@@ -346,5 +343,57 @@ def price_batch(return_stats: bool = False, **columns):
         a = np.broadcast_to(np.asarray(v), (n,))
         for i in range(n):
             setattr(arr[i], k, a[i].item())
-    prices, status, stats = L.price_batch_raw(arr)
-    return (prices, status, stats) if return_stats else (prices, status)
+    return arr
+
+
+def price_batch(return_stats: bool = False, return_info: bool = False, return_qT: bool = False, **columns):
+    """Price many independent options in one GPU call (extension; no
+    counterpart in the reference). Each keyword is a scalar or an array of one
+    value per option; names follow FpeBatchOption. Returns (prices, status),
+    then, when asked, the batch statistics (return_stats), a list of per-option
+    solver records (return_info: status, solve counters, grid) and a list of
+    terminal states q(T) (return_qT; None for in-options)."""
+    arr = _batch_array(columns)
+    prices, status, stats, info, *q = L.price_batch_raw(arr, want_info=True, want_qT=return_qT)
+    out = (prices, status)
+    if return_stats:
+        out += (stats,)
+    if return_info:
+        out += (info,)
+    if return_qT:
+        out += (q[0],)
+    return out
+
+
+def terminal_states(**columns):
+    """q(T) of vanilla or out options (one solve each, on the barrier-truncated
+    domain for out-types) as a list of arrays, with the per-option status."""
+    arr = _batch_array(columns)
+    n = len(arr)
+    ldq = L.MAX_DOF
+    q = np.zeros((n, ldq))
+    nd = np.zeros(n, dtype=np.int32)
+    st = np.zeros(n, dtype=np.int32)
+    stats = L.FpeBatchStats()
+    rc = L.lib().fpe_batch_terminal_states(arr, n, q.ctypes.data_as(L._DP), ldq,
+                                           nd.ctypes.data_as(C.POINTER(C.c_int32)),
+                                           st.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(stats))
+    if rc != 0:
+        raise RuntimeError(f"fpe_batch_terminal_states failed ({rc}): {L.last_error()}")
+    return [q[i, :nd[i]].copy() for i in range(n)], st
+
+
+def operators(**columns):
+    """GPU-assembled M and K (unscaled) of one vanilla or out option as 49-point
+    stencils (N x 49 each) and the grid (n_sb, n_vb)."""
+    arr = _batch_array(columns)
+    cap = L.MAX_DOF
+    M = np.zeros(cap * 49)
+    K = np.zeros(cap * 49)
+    ns, nv = C.c_int32(), C.c_int32()
+    rc = L.lib().fpe_batch_operators(arr, cap, M.ctypes.data_as(L._DP), K.ctypes.data_as(L._DP), C.byref(ns),
+                                     C.byref(nv))
+    if rc != 0:
+        raise RuntimeError(f"fpe_batch_operators failed ({rc}): {L.last_error()}")
+    N = ns.value * nv.value
+    return M[:N * 49].reshape(N, 49), K[:N * 49].reshape(N, 49), (ns.value, nv.value)

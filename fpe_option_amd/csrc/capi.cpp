@@ -34,6 +34,31 @@ fpe::OptionSpec from_batch(const FpeBatchOption& b) {
     return o;
 }
 
+const char* status_text(int st) {
+    switch (st) {
+        case 0: return "ok";
+        case 1: return "invalid parameters";
+        case 2: return "Radau: max steps exceeded";
+        case 3: return "Radau: step size underflow";
+        case 4: return "Radau: non-finite error norm";
+        case 5: return "grid outside the kernels' limits";
+        case 6: return "singular diagonal block in the stage-matrix factorisation";
+        case 7: return "mixed-precision refinement did not reach its tolerance";
+        default: return "unknown status";
+    }
+}
+
+// False (with a message) when any solve of a batch did not succeed.
+bool solves_ok(const std::vector<fpe::SolveResult>& res, std::string& err) {
+    for (size_t k = 0; k < res.size(); ++k)
+        if (res[k].status != 0) {
+            err = std::string("solve ") + std::to_string(k) + " failed: " + status_text(res[k].status) + " (status " +
+                  std::to_string(res[k].status) + ")";
+            return false;
+        }
+    return true;
+}
+
 double* heap_copy(const std::vector<double>& v) {
     if (v.empty()) return nullptr;
     double* p = static_cast<double*>(std::malloc(v.size() * sizeof(double)));
@@ -61,6 +86,9 @@ fpe::OptionSpec make_option(double kappa, double theta, double sigma, double rho
 
 // Prices of one option at several strikes: one solve (two for in-options via
 // parity), one pricing request per strike (src/server/pricer.mojo:82-155).
+// Fails (false, message in err) when a grid does not fit the kernels or a
+// solve does not succeed, as the reference raises from its solver
+// (src/numerics/ode/radau.mojo:257-281,458-464).
 bool price_strikes(const std::vector<fpe::OptionSpec>& opts, const std::vector<double>& K,
                    std::vector<double>& prices, std::string& err) {
     std::vector<fpe::SolveSpec> solves;
@@ -74,22 +102,26 @@ bool price_strikes(const std::vector<fpe::OptionSpec>& opts, const std::vector<d
             fpe::SolveSpec s = fpe::solve_for(o, type);
             solves.push_back(s);
             doms.push_back(fpe::build_domain(s));
+            if (!fpe::grid_fits(doms.back(), &err)) return false;
             for (int k = 0; k < nK; ++k)
                 reqs.push_back({int(solves.size()) - 1, type, type >= 8 ? 0.0 : o.barrier, K[k], sign,
                                 int(a) * nK + k});
+            return true;
         };
+        bool ok;
         if (fpe::is_in_option(t)) {
             const bool call = t % 2 == 0, down = t <= 1;
-            add(call ? 8 : 9, 1.0);
-            add((down ? 2 : 6) + (call ? 0 : 1), -1.0);
+            ok = add(call ? 8 : 9, 1.0) && add((down ? 2 : 6) + (call ? 0 : 1), -1.0);
         } else {
-            add(t, 1.0);
+            ok = add(t, 1.0);
         }
+        if (!ok) return false;
     }
     std::vector<fpe::SolveResult> res;
     fpe::BatchTimes bt;
     fpe::RunOptions ro;
-    return fpe::run_batch(solves, doms, reqs, int(opts.size()) * nK, ro, prices, res, bt, err);
+    if (!fpe::run_batch(solves, doms, reqs, int(opts.size()) * nK, ro, prices, res, bt, err)) return false;
+    return solves_ok(res, err);
 }
 
 }  // namespace
@@ -103,7 +135,7 @@ struct FpeCompute {
     bool have_q0 = false, have_solve = false, have_basis = false;
     std::vector<double> bas[2];
     std::vector<int> bas_c0[2];
-    std::vector<double> q0, pdf;
+    std::vector<double> q0, pdf, qT;
     std::vector<std::vector<double>> hist;
 };
 
@@ -120,11 +152,14 @@ bool ensure_solve(FpeCompute* h) {
         ro.want_q0 = true;
         ro.want_pdf = true;
         ro.hist_cap = cap;
+        ro.want_qT = true;
         if (!fpe::run_batch({h->solve}, {h->dom}, {}, 0, ro, dummy, res, bt, g_last_error)) return false;
+        if (!solves_ok(res, g_last_error)) return false;
         if (res[0].n_acc + 1 <= cap) break;
         cap = res[0].n_acc + 2;
     }
     h->q0 = res[0].q0;
+    h->qT = res[0].qT;
     h->hist = std::move(res[0].hist);
     h->pdf = std::move(res[0].pdf);
     h->have_q0 = h->have_solve = true;
@@ -149,8 +184,7 @@ FpeCompute* fpe_compute_create(double kappa, double theta, double sigma, double 
     h->opt = o;
     h->solve = fpe::solve_for(o, option_type);
     h->dom = fpe::build_domain(h->solve);
-    if (h->dom.n_vb > 64 || h->dom.n_vb < 4 || h->dom.n_sb < 4 || h->dom.n_sb > 300) {
-        g_last_error = "grid outside the kernels' limits";
+    if (!fpe::grid_fits(h->dom, &g_last_error)) {
         delete h;
         return nullptr;
     }
@@ -217,11 +251,26 @@ void fpe_compute_pdf(FpeCompute* ptr, struct FpeMatResult* result) {
     result->n_cols = int32_t(ptr->dom.v_x.size());
 }
 
+// compute_pipeline.mojo:116-151: vanilla and out types price from the cached
+// solve (its terminal state), in-types run the two-solve parity pricer.
 void fpe_compute_price(FpeCompute* ptr, const double* K, int32_t n_K, struct FpeVecResult* result) {
     *result = FpeVecResult{nullptr, 0};
     if (!ptr || n_K < 0) return;
     std::vector<double> k(K, K + n_K), prices;
-    if (!price_strikes({ptr->opt}, k, prices, g_last_error)) return;
+    const int t = ptr->opt.option_type;
+    if (fpe::is_in_option(t)) {
+        if (!price_strikes({ptr->opt}, k, prices, g_last_error)) return;
+    } else {
+        if (!ensure_solve(ptr)) return;
+        std::vector<fpe::Request> reqs;
+        for (int j = 0; j < n_K; ++j) reqs.push_back({0, t, t >= 8 ? 0.0 : ptr->opt.barrier, k[j], 1.0, j});
+        std::vector<std::vector<double>> given(1, ptr->qT);
+        std::vector<fpe::SolveResult> res;
+        fpe::BatchTimes bt;
+        fpe::RunOptions ro;
+        ro.given_qT = &given;
+        if (!fpe::run_batch({ptr->solve}, {ptr->dom}, reqs, n_K, ro, prices, res, bt, g_last_error)) return;
+    }
     result->data = heap_copy(prices);
     result->len = n_K;
 }
@@ -341,10 +390,46 @@ void fpe_batch_option_default(FpeBatchOption* o) {
     o->rtol = 1e-4, o->atol = 1e-6, o->knot_width_s = 0.1, o->knot_width_v = 0.1;
 }
 
-int32_t fpe_batch_price(const FpeBatchOption* opts, int32_t n, double* prices, int32_t* status,
-                        FpeBatchStats* stats) {
+namespace {
+
+void fill_stats(const std::vector<fpe::SolveResult>& res, const fpe::BatchTimes& bt, FpeBatchStats* stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->n_solves = int64_t(res.size());
+    for (auto& r : res) {
+        stats->n_accepted += r.n_acc, stats->n_rejected += r.n_rej, stats->n_lu += r.n_lu;
+        stats->n_newton += r.n_newt, stats->n_err_solves += r.n_err, stats->n_inner_iters += r.n_inner;
+        stats->n_admm_iters += r.n_admm;
+        stats->n_refine += r.n_refine;
+    }
+    for (auto& r : res) {
+        const double nb = r.nsb, m = r.nvb, N = nb * m;
+        const double lu = 24.67 * nb * m * m * m, sol = 14.0 * nb * m * m, spmv = 98.0 * N;
+        const double fac_bytes = 7.0 * nb * m * m * 8.0;
+        // direct: real + complex LU per refresh; GMRES: real LU and the solves of the inner iterations
+        const bool direct = r.direct;
+        stats->alg_flops += r.n_lu * (direct ? 5.0 : 1.0) * lu +
+                            r.n_newt * ((direct ? 5.0 : 3.0) * sol + 6.0 * spmv) +
+                            r.n_inner * (2.0 * sol + 4.0 * spmv) + r.n_err * (sol + 2.0 * spmv);
+        stats->alg_bytes += r.n_lu * (direct ? 3.0 : 1.0) * 2.0 * fac_bytes +
+                            r.n_newt * (direct ? 3.0 : 1.0) * fac_bytes + r.n_inner * fac_bytes + r.n_err * fac_bytes;
+    }
+    for (auto& r : res) {
+        stats->wg_s_assemble += r.tk[0], stats->wg_s_lu += r.tk[1], stats->wg_s_solve += r.tk[2];
+        stats->wg_s_spmv += r.tk[3], stats->wg_s_other += r.tk[4], stats->wg_s_total += r.tk[5];
+        stats->wg_s_lu_inverse += r.tk[6], stats->wg_s_lu_stage += r.tk[7];
+        stats->wg_s_lu_lmul += r.tk[8], stats->wg_s_lu_schur += r.tk[9];
+    }
+    stats->ms_setup = bt.setup, stats->ms_assemble = bt.assemble, stats->ms_ic = bt.ic;
+    stats->ms_radau = bt.radau, stats->ms_payoff = bt.payoff, stats->ms_total = bt.total;
+    stats->n_radau_slots = bt.n_slots;
+}
+
+}  // namespace
+
+int32_t fpe_batch_price_detail(const FpeBatchOption* opts, int32_t n, double* prices, int32_t* status,
+                               FpeSolveInfo* info, double* qT, int64_t ldq, FpeBatchStats* stats) {
     g_last_error.clear();
-    if (n < 0 || (n > 0 && (!opts || !prices || !status))) {
+    if (n < 0 || (n > 0 && (!opts || !prices || !status)) || (qT && ldq <= 0)) {
         g_last_error = "invalid arguments";
         return 1;
     }
@@ -353,42 +438,39 @@ int32_t fpe_batch_price(const FpeBatchOption* opts, int32_t n, double* prices, i
     std::vector<double> p;
     std::vector<int> s;
     std::vector<fpe::SolveResult> res;
+    std::vector<std::vector<int>> used;
     fpe::BatchTimes bt;
-    if (!fpe::price_options(v, p, s, &res, &bt, g_last_error)) return 2;
+    if (!fpe::price_options(v, p, s, &res, &bt, g_last_error, &used, qT != nullptr)) return 2;
     for (int i = 0; i < n; ++i) prices[i] = p[i], status[i] = s[i];
-    if (stats) {
-        std::memset(stats, 0, sizeof(*stats));
-        stats->n_solves = int64_t(res.size());
-        for (auto& r : res) {
-            stats->n_accepted += r.n_acc, stats->n_rejected += r.n_rej, stats->n_lu += r.n_lu;
-            stats->n_newton += r.n_newt, stats->n_err_solves += r.n_err, stats->n_inner_iters += r.n_inner;
-            stats->n_admm_iters += r.n_admm;
-            stats->n_refine += r.n_refine;
+    if (qT) {  // the terminal state of a single-solve option
+        for (int i = 0; i < n; ++i) {
+            if (used[i].size() != 1) continue;
+            const std::vector<double>& q = res[used[i][0]].qT;
+            if ((int64_t)q.size() <= ldq) std::copy(q.begin(), q.end(), qT + (int64_t)i * ldq);
         }
-        for (auto& r : res) {
-            const double nb = r.nsb, m = r.nvb, N = nb * m;
-            const double lu = 24.67 * nb * m * m * m, sol = 14.0 * nb * m * m, spmv = 98.0 * N;
-            const double fac_bytes = 7.0 * nb * m * m * 8.0;
-            // direct: real + complex LU per refresh; GMRES: real LU and the solves of the inner iterations
-            const bool direct = r.direct;
-            stats->alg_flops += r.n_lu * (direct ? 5.0 : 1.0) * lu +
-                                r.n_newt * ((direct ? 5.0 : 3.0) * sol + 6.0 * spmv) +
-                                r.n_inner * (2.0 * sol + 4.0 * spmv) + r.n_err * (sol + 2.0 * spmv);
-            stats->alg_bytes += r.n_lu * (direct ? 3.0 : 1.0) * 2.0 * fac_bytes +
-                                r.n_newt * (direct ? 3.0 : 1.0) * fac_bytes + r.n_inner * fac_bytes +
-                                r.n_err * fac_bytes;
-        }
-        for (auto& r : res) {
-            stats->wg_s_assemble += r.tk[0], stats->wg_s_lu += r.tk[1], stats->wg_s_solve += r.tk[2];
-            stats->wg_s_spmv += r.tk[3], stats->wg_s_other += r.tk[4], stats->wg_s_total += r.tk[5];
-            stats->wg_s_lu_inverse += r.tk[6], stats->wg_s_lu_stage += r.tk[7];
-            stats->wg_s_lu_lmul += r.tk[8], stats->wg_s_lu_schur += r.tk[9];
-        }
-        stats->ms_setup = bt.setup, stats->ms_assemble = bt.assemble, stats->ms_ic = bt.ic;
-        stats->ms_radau = bt.radau, stats->ms_payoff = bt.payoff, stats->ms_total = bt.total;
-        stats->n_radau_slots = bt.n_slots;
     }
+    if (info) {
+        for (int i = 0; i < n; ++i) {
+            FpeSolveInfo& f = info[i];
+            std::memset(&f, 0, sizeof(f));
+            f.status = s[i];
+            f.n_solves = int32_t(used[i].size());
+            for (int k : used[i]) {
+                const fpe::SolveResult& r = res[k];
+                f.n_accepted += r.n_acc, f.n_rejected += r.n_rej, f.n_lu += r.n_lu, f.n_newton += r.n_newt;
+                f.n_err_solves += r.n_err, f.n_inner_iters += r.n_inner, f.n_admm_iters += r.n_admm;
+                f.n_refine += r.n_refine;
+                if (f.n_sb == 0) f.n_sb = r.nsb, f.n_vb = r.nvb;
+            }
+        }
+    }
+    if (stats) fill_stats(res, bt, stats);
     return 0;
+}
+
+int32_t fpe_batch_price(const FpeBatchOption* opts, int32_t n, double* prices, int32_t* status,
+                        FpeBatchStats* stats) {
+    return fpe_batch_price_detail(opts, n, prices, status, nullptr, nullptr, 0, stats);
 }
 
 void fpe_compute_basis_1d(FpeCompute* ptr, int32_t axis, int32_t deriv, struct FpeCsrResult* result) {
@@ -447,9 +529,14 @@ int32_t fpe_batch_terminal_states(const FpeBatchOption* opts, int32_t n, double*
         status[i] = 1;
         fpe::OptionSpec o = from_batch(opts[i]);
         std::string why;
-        if ((o.option_type != 8 && o.option_type != 9) || !fpe::option_is_valid(o, &why)) continue;
+        // one solve per option: vanilla on the full domain, out-types on the truncated one
+        if (fpe::is_in_option(o.option_type) || !fpe::option_is_valid(o, &why)) continue;
         fpe::SolveSpec sp = fpe::solve_for(o, o.option_type);
         fpe::Domain d = fpe::build_domain(sp);
+        if (!fpe::grid_fits(d, nullptr)) {
+            status[i] = 5;
+            continue;
+        }
         if ((int64_t)d.n_sb * d.n_vb > ldq) continue;
         which[i] = int(solves.size());
         solves.push_back(sp);
@@ -468,18 +555,40 @@ int32_t fpe_batch_terminal_states(const FpeBatchOption* opts, int32_t n, double*
         n_dof[i] = int32_t(r.qT.size());
         std::copy(r.qT.begin(), r.qT.end(), qT + (int64_t)i * ldq);
     }
-    if (stats) {
-        std::memset(stats, 0, sizeof(*stats));
-        stats->n_solves = int64_t(res.size());
-        for (auto& r : res) {
-            stats->n_accepted += r.n_acc, stats->n_rejected += r.n_rej, stats->n_lu += r.n_lu;
-            stats->n_newton += r.n_newt, stats->n_err_solves += r.n_err, stats->n_inner_iters += r.n_inner;
-            stats->n_admm_iters += r.n_admm;
-        }
-        stats->ms_setup = bt.setup, stats->ms_assemble = bt.assemble, stats->ms_ic = bt.ic;
-        stats->ms_radau = bt.radau, stats->ms_payoff = bt.payoff, stats->ms_total = bt.total;
-        stats->n_radau_slots = bt.n_slots;
+    if (stats) fill_stats(res, bt, stats);
+    return 0;
+}
+
+int32_t fpe_batch_operators(const FpeBatchOption* opt, int32_t cap_dof, double* M49, double* K49, int32_t* n_sb,
+                            int32_t* n_vb) {
+    g_last_error.clear();
+    if (!opt || !M49 || !K49 || !n_sb || !n_vb) {
+        g_last_error = "invalid arguments";
+        return 1;
     }
+    fpe::OptionSpec o = from_batch(*opt);
+    std::string why;
+    if (fpe::is_in_option(o.option_type) || !fpe::option_is_valid(o, &why)) {
+        g_last_error = "a valid vanilla or out option is required: " + why;
+        return 1;
+    }
+    fpe::SolveSpec sp = fpe::solve_for(o, o.option_type);
+    fpe::Domain d = fpe::build_domain(sp);
+    *n_sb = d.n_sb, *n_vb = d.n_vb;
+    if (!fpe::grid_fits(d, &g_last_error)) return 5;
+    if ((int64_t)d.n_sb * d.n_vb > cap_dof) {
+        g_last_error = "capacity too small";
+        return 1;
+    }
+    std::vector<fpe::SolveResult> res;
+    fpe::BatchTimes bt;
+    std::vector<double> dummy;
+    fpe::RunOptions ro;
+    ro.want_stencils = true;
+    ro.stop_after_ic = true;
+    if (!fpe::run_batch({sp}, {d}, {}, 0, ro, dummy, res, bt, g_last_error)) return 2;
+    std::copy(res[0].M49.begin(), res[0].M49.end(), M49);
+    std::copy(res[0].K49.begin(), res[0].K49.end(), K49);
     return 0;
 }
 

@@ -56,6 +56,12 @@ struct Inst {
     double ic_sig_s, ic_sig_v, rtol, atol;
 };
 
+// Per-solve status codes (the FPE_* codes of the C ABI).
+enum {
+    ST_OK = 0, ST_INVALID = 1, ST_MAX_STEPS = 2, ST_H_UNDERFLOW = 3, ST_NONFINITE = 4, ST_TOO_LARGE = 5,
+    ST_SINGULAR = 6, ST_REFINE_FAIL = 7
+};
+
 // Per-instance result counters.
 struct InstOut {
     int status, n_acc, n_rej, n_lu, n_newt, n_err, n_inner, n_admm;

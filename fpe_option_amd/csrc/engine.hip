@@ -88,6 +88,20 @@ bool upload(DBuf& b, const std::vector<T>& h, std::string& err) {
 
 }  // namespace
 
+bool grid_fits(const Domain& d, std::string* why) {
+    if (d.n_vb < 4 || d.n_vb > MAX_BLOCK || d.n_sb < 4 || d.n_sb > MAX_SB) {
+        if (why)
+            *why = "grid outside the kernels' limits (n_vb in [4, " + std::to_string(MAX_BLOCK) + "], n_sb in [4, " +
+                   std::to_string(MAX_SB) + "]): n_sb=" + std::to_string(d.n_sb) + " n_vb=" + std::to_string(d.n_vb);
+        return false;
+    }
+    if (admm_lds_bytes(d.n_sb, d.n_vb, false) > ADMM_LDS_MAX) {
+        if (why) *why = "grid too large for the initial-condition kernel's LDS budget";
+        return false;
+    }
+    return true;
+}
+
 int device_count() {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;
@@ -121,13 +135,9 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
         I.nsb = d.n_sb, I.nvb = d.n_vb;
         I.nqs = int(d.s_x.size()), I.nqv = int(d.v_x.size());
         I.nsk = int(d.s_knots.size()), I.nvk = int(d.v_knots.size());
-        if (I.nvb < 4 || I.nvb > MAX_BLOCK || I.nsb < 4 || I.nsb > 300) {
-            err = "grid too large for the kernels (n_vb must be in [4, 64], n_sb in [4, 300]): n_sb=" +
-                  std::to_string(I.nsb) + " n_vb=" + std::to_string(I.nvb);
-            return false;
-        }
-        if (admm_lds_bytes(I.nsb, I.nvb, false) > ADMM_LDS_MAX) {
-            err = "grid too large for the initial-condition kernel's LDS budget";
+        if (!grid_fits(d, &err)) return false;  // callers screen grids per option first
+        if (opt.given_qT && (*opt.given_qT)[s].size() != size_t(I.nsb) * I.nvb) {
+            err = "given terminal state has the wrong length";
             return false;
         }
         I.s_lneu = sp.s_left_neumann, I.s_rneu = sp.s_right_neumann;
@@ -228,34 +238,48 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
     times.setup = std::chrono::duration<double, std::milli>(t_setup - t_start).count();
 
     // ---- kernels
+    // given_qT: price (and PDF) from terminal states the caller already holds; only
+    // the basis tables are rebuilt, then the states are copied into their slots.
+    const bool given = opt.given_qT != nullptr;
+    const bool solved = !opt.stop_after_ic;
     HIP_OK(hipEventRecord(G.ev[0], st));
     hipLaunchKernelGGL(k_basis, dim3(n, 2), dim3(NT), 0, st, dI, G.kn.as<double>(), G.qd.as<double>(),
                        G.bv.as<double>(), G.bc.as<int>());
-    hipLaunchKernelGGL(k_factors, dim3(n, 2), dim3(NT), 0, st, dI, G.qd.as<double>(), G.bv.as<double>(),
-                       G.bc.as<int>(), G.fc.as<double>());
-    const int chunks = int(std::min<long long>(64, (max_N + NT - 1) / NT));
-    hipLaunchKernelGGL(k_stencil, dim3(n, chunks), dim3(NT), 0, st, dI, G.fc.as<double>(), G.st.as<double>(),
-                       G.vc.as<double>());
+    if (!given) {
+        hipLaunchKernelGGL(k_factors, dim3(n, 2), dim3(NT), 0, st, dI, G.qd.as<double>(), G.bv.as<double>(),
+                           G.bc.as<int>(), G.fc.as<double>());
+        const int chunks = int(std::min<long long>(64, (max_N + NT - 1) / NT));
+        hipLaunchKernelGGL(k_stencil, dim3(n, chunks), dim3(NT), 0, st, dI, G.fc.as<double>(), G.st.as<double>(),
+                           G.vc.as<double>());
+    } else {
+        for (int s = 0; s < n; ++s) {
+            const long long N = (long long)hi[s].nsb * hi[s].nvb;
+            HIP_OK(hipMemcpyAsync(G.vc.as<double>() + hi[s].o_vec + V_QT * N, (*opt.given_qT)[s].data(),
+                                  sizeof(double) * N, hipMemcpyHostToDevice, st));
+        }
+    }
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord(G.ev[1], st));
-    hipLaunchKernelGGL(k_ic_rhs, dim3(n), dim3(NT), 0, st, dI, G.order.as<int>(), G.qd.as<double>(), G.bv.as<double>(),
-                       G.bc.as<int>(), G.vc.as<double>());
-    hipLaunchKernelGGL(k_eig, dim3(n), dim3(ENT), EIG_LDS_S, st, dI, G.order.as<int>(), 0, EIG_LDS_S,
-                       G.fc.as<double>(), G.eg.as<double>());
-    hipLaunchKernelGGL(k_eig, dim3(n), dim3(ENT), EIG_LDS_V, st, dI, G.order.as<int>(), 1, EIG_LDS_V,
-                       G.fc.as<double>(), G.eg.as<double>());
-    hipLaunchKernelGGL(k_admm, dim3(n), dim3(ANT), ADMM_LDS_MAX, st, dI, G.order.as<int>(), G.st.as<double>(), G.eg.as<double>(),
-                       G.vc.as<double>(), G.out.as<InstOut>());
-    HIP_OK(hipGetLastError());
+    if (!given) {
+        hipLaunchKernelGGL(k_ic_rhs, dim3(n), dim3(NT), 0, st, dI, G.order.as<int>(), G.qd.as<double>(),
+                           G.bv.as<double>(), G.bc.as<int>(), G.vc.as<double>());
+        hipLaunchKernelGGL(k_eig, dim3(n), dim3(ENT), EIG_LDS_S, st, dI, G.order.as<int>(), 0, EIG_LDS_S,
+                           G.fc.as<double>(), G.eg.as<double>());
+        hipLaunchKernelGGL(k_eig, dim3(n), dim3(ENT), EIG_LDS_V, st, dI, G.order.as<int>(), 1, EIG_LDS_V,
+                           G.fc.as<double>(), G.eg.as<double>());
+        hipLaunchKernelGGL(k_admm, dim3(n), dim3(ANT), ADMM_LDS_MAX, st, dI, G.order.as<int>(), G.st.as<double>(),
+                           G.eg.as<double>(), G.vc.as<double>(), G.out.as<InstOut>());
+        HIP_OK(hipGetLastError());
+    }
     HIP_OK(hipEventRecord(G.ev[2], st));
-    if (!opt.stop_after_ic) {
-        hipLaunchKernelGGL(k_radau, dim3(n_slots), dim3(RNT), 0, st, dI, n, G.queue.as<int>(), G.order.as<int>(), G.st.as<double>(),
-                           G.vc.as<double>(), G.ws.as<double>(), G.wi.as<int>(), L, G.hs.as<double>(),
-                           G.out.as<InstOut>());
+    if (solved && !given) {
+        hipLaunchKernelGGL(k_radau, dim3(n_slots), dim3(RNT), 0, st, dI, n, G.queue.as<int>(), G.order.as<int>(),
+                           G.st.as<double>(), G.vc.as<double>(), G.ws.as<double>(), G.wi.as<int>(), L,
+                           G.hs.as<double>(), G.out.as<InstOut>());
         HIP_OK(hipGetLastError());
     }
     HIP_OK(hipEventRecord(G.ev[3], st));
-    if (!opt.stop_after_ic && !reqs.empty()) {
+    if (solved && !reqs.empty()) {
         hipLaunchKernelGGL(k_marginal, dim3(n), dim3(NT), 0, st, dI, G.qd.as<double>(), G.bv.as<double>(),
                            G.bc.as<int>(), G.vc.as<double>(), G.ow.as<long long>(), G.wm.as<double>());
         hipLaunchKernelGGL(k_price, dim3(int(reqs.size())), dim3(NT), 0, st, dI, G.qd.as<double>(),
@@ -308,6 +332,26 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
             HIP_OK(hipMemcpy(R.q0.data(), G.vc.as<double>() + hi[s].o_vec + V_Q0 * N, sizeof(double) * N,
                              hipMemcpyDeviceToHost));
         }
+        if (opt.want_stencils && !given) {
+            // unscaled M and K as N x 49 rows (entry (dJ + 3) * 7 + dj + 3), from the
+            // D-scaled plane-major stencils: M[r][e] = D[r] Ms[r][e] D[col(r, e)]
+            std::vector<double> st98(98 * N), Dh(N);
+            HIP_OK(hipMemcpy(st98.data(), G.st.as<double>() + hi[s].o_st, sizeof(double) * 98 * N,
+                             hipMemcpyDeviceToHost));
+            HIP_OK(hipMemcpy(Dh.data(), G.vc.as<double>() + hi[s].o_vec + V_D * N, sizeof(double) * N,
+                             hipMemcpyDeviceToHost));
+            const int nvb = hi[s].nvb;
+            R.M49.assign(49 * N, 0.0);
+            R.K49.assign(49 * N, 0.0);
+            for (long long r = 0; r < N; ++r)
+                for (int e = 0; e < 49; ++e) {
+                    const long long c = r + (long long)(e / 7 - 3) * nvb + (e % 7 - 3);
+                    const int I = int(r / nvb), i = int(r % nvb), J = I + e / 7 - 3, j = i + e % 7 - 3;
+                    if (J < 0 || J >= hi[s].nsb || j < 0 || j >= nvb) continue;
+                    R.M49[r * 49 + e] = Dh[r] * st98[e * N + r] * Dh[c];
+                    R.K49[r * 49 + e] = Dh[r] * st98[(49 + e) * N + r] * Dh[c];
+                }
+        }
         if (opt.want_qT && !opt.stop_after_ic) {
             R.qT.resize(N);
             HIP_OK(hipMemcpy(R.qT.data(), G.vc.as<double>() + hi[s].o_vec + V_QT * N, sizeof(double) * N,
@@ -337,7 +381,8 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
 }
 
 bool price_options(const std::vector<OptionSpec>& opts, std::vector<double>& prices, std::vector<int>& status,
-                   std::vector<SolveResult>* results_out, BatchTimes* times_out, std::string& err) {
+                   std::vector<SolveResult>* results_out, BatchTimes* times_out, std::string& err,
+                   std::vector<std::vector<int>>* used_out, bool want_qT) {
     const int n = int(opts.size());
     prices.assign(n, 0.0);
     status.assign(n, 0);
@@ -345,14 +390,12 @@ bool price_options(const std::vector<OptionSpec>& opts, std::vector<double>& pri
     std::vector<Domain> doms;
     std::vector<Request> reqs;
     std::vector<std::vector<int>> used(n);
-    auto add_solve = [&](const OptionSpec& o, int type, double sign, int out) {
-        SolveSpec s = solve_for(o, type);
-        solves.push_back(s);
-        doms.push_back(build_domain(s));
-        Request r{int(solves.size()) - 1, type, o.barrier, o.strike, sign, out};
-        if (type >= 8) r.barrier = 0.0;
-        reqs.push_back(r);
-        used[out].push_back(int(solves.size()) - 1);
+    // an option's solves: (spec, domain, payoff type, sign)
+    struct Part {
+        SolveSpec s;
+        Domain d;
+        int type;
+        double sign;
     };
     for (int i = 0; i < n; ++i) {
         const OptionSpec& o = opts[i];
@@ -362,26 +405,49 @@ bool price_options(const std::vector<OptionSpec>& opts, std::vector<double>& pri
             continue;
         }
         const int t = o.option_type;
-        if (is_in_option(t)) {
+        std::vector<std::pair<OptionSpec, std::pair<int, double>>> want;
+        if (is_in_option(t)) {  // in = vanilla (full domain) - out (truncated)
             const bool call = t % 2 == 0, down = t <= 1;
             OptionSpec van = o;
             van.barrier = 0.0;
-            add_solve(van, call ? 8 : 9, 1.0, i);
-            add_solve(o, (down ? 2 : 6) + (call ? 0 : 1), -1.0, i);
+            want.push_back({van, {call ? 8 : 9, 1.0}});
+            want.push_back({o, {(down ? 2 : 6) + (call ? 0 : 1), -1.0}});
         } else {
-            add_solve(o, t, 1.0, i);
+            want.push_back({o, {t, 1.0}});
+        }
+        std::vector<Part> parts;
+        bool fits = true;
+        for (auto& w : want) {
+            Part p{solve_for(w.first, w.second.first), Domain(), w.second.first, w.second.second};
+            p.d = build_domain(p.s);
+            fits = fits && grid_fits(p.d, nullptr);
+            parts.push_back(std::move(p));
+        }
+        if (!fits) {  // one oversized option does not stop the rest of the batch
+            status[i] = 5;
+            continue;
+        }
+        for (auto& p : parts) {
+            solves.push_back(p.s);
+            doms.push_back(std::move(p.d));
+            Request r{int(solves.size()) - 1, p.type, opts[i].barrier, opts[i].strike, p.sign, i};
+            if (p.type >= 8) r.barrier = 0.0;
+            reqs.push_back(r);
+            used[i].push_back(int(solves.size()) - 1);
         }
     }
     std::vector<SolveResult> res;
     BatchTimes bt;
     RunOptions ro;
+    ro.want_qT = want_qT;
     if (!run_batch(solves, doms, reqs, n, ro, prices, res, bt, err)) return false;
     for (int i = 0; i < n; ++i) {
         for (int s : used[i])
             if (res[s].status != 0 && status[i] == 0) status[i] = res[s].status;
-        if (status[i] != 0) prices[i] = status[i] == 1 ? 0.0 : prices[i];
+        if (status[i] == 1 || status[i] == 5) prices[i] = 0.0;
     }
     if (results_out) *results_out = std::move(res);
+    if (used_out) *used_out = std::move(used);
     if (times_out) *times_out = bt;
     return true;
 }

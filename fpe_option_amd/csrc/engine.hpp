@@ -18,6 +18,7 @@ struct SolveResult {
     std::vector<double> q0, qT;              // filled when requested
     std::vector<std::vector<double>> hist;   // accepted states (stepwise API)
     std::vector<double> pdf;                 // n_qs x n_qv (stepwise API)
+    std::vector<double> M49, K49;            // unscaled stencils, N x 49 (operator export)
     std::vector<double> bas[2];              // per-point basis tables (8 doubles) for s, v
     std::vector<int> bas_c0[2];              // first recombined column per point
 };
@@ -35,10 +36,20 @@ struct Request {
 };
 
 struct RunOptions {
-    bool want_q0 = false, want_qT = false, want_pdf = false, want_basis = false;
+    bool want_q0 = false, want_qT = false, want_pdf = false, want_basis = false, want_stencils = false;
     int hist_cap = 0;  // > 0: keep up to this many accepted states per solve
     bool stop_after_ic = false;
+    // when set (one state of N values per solve): no solve runs; prices and PDFs
+    // are computed from these terminal states
+    const std::vector<std::vector<double>>* given_qT = nullptr;
 };
+
+// Largest block-row count the kernels take (the v block width is bounded by MAX_BLOCK = 64).
+constexpr int MAX_SB = 300;
+
+// Whether a discretised domain fits the kernels' limits (block width, block
+// rows, initial-condition LDS budget); why receives the reason when not.
+bool grid_fits(const Domain& d, std::string* why);
 
 // Runs solves + pricing requests; out_prices has n_out entries (accumulated).
 // Returns false with a message in err on failure.
@@ -47,8 +58,11 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
                std::vector<SolveResult>& results, BatchTimes& times, std::string& err);
 
 // Price options (with in-out parity); status per option.
+// Options whose grid does not fit get status FPE_TOO_LARGE (5) and price 0;
+// used (when given) receives the indices of each option's solves in results.
 bool price_options(const std::vector<OptionSpec>& opts, std::vector<double>& prices, std::vector<int>& status,
-                   std::vector<SolveResult>* results, BatchTimes* times, std::string& err);
+                   std::vector<SolveResult>* results, BatchTimes* times, std::string& err,
+                   std::vector<std::vector<int>>* used = nullptr, bool want_qT = false);
 
 int device_count();
 

@@ -134,7 +134,8 @@ __device__ void newton_mv(const double* __restrict__ Ms, const double* __restric
 }
 
 // GMRES operator: [Ed -B M; B M Ed] [vr; vi] with Ed = ALPH M + h K formed
-// entry-wise as in radau.mojo:757-831,963-992.
+// entry-wise as in radau.mojo:757-831,963-992; h is the step size of the last
+// refactorisation.
 __device__ void gmres_mv(const double* __restrict__ Ms, const double* __restrict__ Ks, int ns, int nv, double alph,
                          double beta, double h, const double* vr, const double* vi, double* wr, double* wi) {
     const int N = ns * nv;
@@ -289,6 +290,7 @@ __global__ void __launch_bounds__(RNT, RADAU_WG_PER_CU * RNW / 4) k_radau(const 
         constexpr int REFINE_MAX = 10;
         constexpr double REFINE_TOL = 1e-12;
         int n_ref = 0;
+        bool refine_fail = false;  // a stage solve left its refinement above REFINE_TOL
         auto solve_real = [&](double* x) {
             if (!mixed) {
                 block_solve1<false, double>(P, nullptr, ns, m, x, nullptr, ring);
@@ -300,12 +302,16 @@ __global__ void __launch_bounds__(RNT, RADAU_WG_PER_CU * RNW / 4) k_radau(const 
             for (int r = tid; r < N; r += RNT) b[r] = x[r], bn = fmax(bn, fabs(x[r]));
             bn = block_max_w<RNW>(bn, S.red);
             block_solve1<false, float>(Pf, nullptr, ns, m, x, nullptr, ring);
-            for (int it = 0; it < REFINE_MAX; ++it) {
+            for (int it = 0;; ++it) {
                 stage_residual<false>(Ms, Ks, ns, nv, U1, 0.0, h_lu, x, nullptr, b, nullptr, tv, nullptr);
                 double rn = 0.0;
                 for (int r = tid; r < N; r += RNT) rn = fmax(rn, fabs(tv[r]));
                 rn = block_max_w<RNW>(rn, S.red);
                 if (!(rn > REFINE_TOL * bn)) break;
+                if (it == REFINE_MAX) {
+                    refine_fail = true;
+                    break;
+                }
                 ++n_ref;
                 block_solve1<false, float>(Pf, nullptr, ns, m, tv, nullptr, ring);
                 for (int r = tid; r < N; r += RNT) x[r] += tv[r];
@@ -328,12 +334,16 @@ __global__ void __launch_bounds__(RNT, RADAU_WG_PER_CU * RNW / 4) k_radau(const 
             }
             bn = block_max_w<RNW>(bn, S.red);
             block_solve1<true, float>(Crf, Cif, ns, m, xr, xi, ring);
-            for (int it = 0; it < REFINE_MAX; ++it) {
+            for (int it = 0;; ++it) {
                 stage_residual<true>(Ms, Ks, ns, nv, ALPH, BETA, h_lu, xr, xi, br, bi, tr, ti);
                 double rn = 0.0;
                 for (int r = tid; r < N; r += RNT) rn = fmax(rn, fmax(fabs(tr[r]), fabs(ti[r])));
                 rn = block_max_w<RNW>(rn, S.red);
                 if (!(rn > REFINE_TOL * bn)) break;
+                if (it == REFINE_MAX) {
+                    refine_fail = true;
+                    break;
+                }
                 ++n_ref;
                 block_solve1<true, float>(Crf, Cif, ns, m, tr, ti, ring);
                 for (int r = tid; r < N; r += RNT) xr[r] += tr[r], xi[r] += ti[r];
@@ -342,9 +352,9 @@ __global__ void __launch_bounds__(RNT, RADAU_WG_PER_CU * RNW / 4) k_radau(const 
         };
 
         while (posneg * (t1 - t) > uround * fmax(fabs(t), fabs(t1))) {
-            if (n_steps > 100000 || ++guard > 2000000) { status = 2; break; }
+            if (n_steps > 100000 || ++guard > 2000000) { status = ST_MAX_STEPS; break; }
             if (posneg * (t + 1.01 * h - t1) > 0.0) h = t1 - t;
-            if (!(fabs(h) >= 1e-14)) { status = 3; break; }
+            if (!(fabs(h) >= 1e-14)) { status = ST_H_UNDERFLOW; break; }
             bool need_lu = (h_lu == 0.0);
             if (!need_lu) {
                 const double ratio = fabs(h / h_lu);
@@ -378,9 +388,12 @@ __global__ void __launch_bounds__(RNT, RADAU_WG_PER_CU * RNW / 4) k_radau(const 
                         lap(tk_lu);
                     }
                 }
-                (void)bad;
                 h_lu = h;
                 ++n_lu;
+                if (bad) {  // a zero or non-finite pivot: the factors cannot be used
+                    status = ST_SINGULAR;
+                    break;
+                }
             }
             // scal from y; w = K y
             for (int r = tid; r < N; r += RNT) vv(S_SCAL)[r] = sa + sr * fabs(y[r]);
@@ -459,7 +472,9 @@ __global__ void __launch_bounds__(RNT, RADAU_WG_PER_CU * RNW / 4) k_radau(const 
                         int j = 0;
                         while (j < GM_K) {
                             tk_mark = wall_clock64();
-                            gmres_mv(Ms, Ks, ns, nv, ALPH, BETA, h, vv(S_VRE0 + j), vv(S_VIM0 + j), vv(S_WRE),
+                            // E_diag is the one built at the last refactorisation (h_lu), as
+                            // the reference's E_diag_cached (radau.mojo:289-297,362-364)
+                            gmres_mv(Ms, Ks, ns, nv, ALPH, BETA, h_lu, vv(S_VRE0 + j), vv(S_VIM0 + j), vv(S_WRE),
                                      vv(S_WIM));
                             __syncthreads();
                             lap(tk_mv);
@@ -598,7 +613,7 @@ __global__ void __launch_bounds__(RNT, RADAU_WG_PER_CU * RNW / 4) k_radau(const 
             if (newt_fail || !converged) {
                 reject = true;
                 h = first ? h * 0.1 : h * 0.5;
-                if (!(fabs(h) >= 1e-14)) { status = 3; break; }
+                if (!(fabs(h) >= 1e-14)) { status = ST_H_UNDERFLOW; break; }
                 continue;
             }
             theta = theta_loc;
@@ -650,7 +665,7 @@ __global__ void __launch_bounds__(RNT, RADAU_WG_PER_CU * RNW / 4) k_radau(const 
                 err_norm = sqrt(en / double(N));
                 if (err_norm < 1e-10) err_norm = 1e-10;
             }
-            if (!(err_norm == err_norm) || isinf(err_norm)) { status = 4; break; }
+            if (!(err_norm == err_norm) || isinf(err_norm)) { status = ST_NONFINITE; break; }
             const double fac = fmin(safety, cfac / double(newt + 2 * nit));
             double quot = fmax(1.0 / fac2, fmin(1.0 / fac1, sqrt(sqrt(err_norm)) / fac));
             double h_new = h / quot;
@@ -690,7 +705,9 @@ __global__ void __launch_bounds__(RNT, RADAU_WG_PER_CU * RNW / 4) k_radau(const 
         for (int r = tid; r < N; r += RNT) qT[r] = Dinv[r] * y[r];
         if (tid == 0) {
             InstOut& o = out[id];
-            o.status = status;
+            // a refinement that stopped above its tolerance is reported, the
+            // solve itself ran to T
+            o.status = (status == ST_OK && refine_fail) ? ST_REFINE_FAIL : status;
             o.n_acc = n_acc, o.n_rej = n_rej, o.n_lu = n_lu, o.n_newt = n_newt, o.n_err = n_err;
             o.n_inner = n_inner, o.n_hist = nh, o.t_end = t, o.h_last = h;
             o.n_refine = n_ref;

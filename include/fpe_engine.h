@@ -140,7 +140,9 @@ void fpe_compute_free_oneshot(struct FpeOneshotResult* r);
 #define FPE_MAX_STEPS 2    /* Radau exceeded 100000 steps */
 #define FPE_H_UNDERFLOW 3  /* step size fell below 1e-14 */
 #define FPE_NONFINITE 4    /* NaN/Inf in the solution */
-#define FPE_TOO_LARGE 5    /* grid exceeds the kernels' limits */
+#define FPE_TOO_LARGE 5    /* grid exceeds the kernels' limits (n_vb > 64, n_sb > 300); the rest of the batch runs */
+#define FPE_SINGULAR 6     /* zero or non-finite pivot in a diagonal block of a stage matrix */
+#define FPE_REFINE_FAIL 7  /* FPE_PREC_MIXED: a stage solve's FP64 refinement stopped above 1e-12 (price kept) */
 
 /** One option of a batch: parameters, one strike, type and solver options. */
 typedef struct FpeBatchOption {
@@ -203,18 +205,43 @@ void fpe_compute_free_csr(struct FpeCsrResult* r);
 void fpe_batch_option_default(FpeBatchOption* o);
 
 /** Price n independent options on the GPU (in-options via in-out parity).
- *  prices[i] is the discounted price; status[i] an FPE_* code.
+ *  prices[i] is the discounted price; status[i] an FPE_* code (an option whose
+ *  grid does not fit gets FPE_TOO_LARGE and price 0; the others still run).
  *  Returns 0 on success, nonzero if the batch could not run (see fpe_last_error). */
 int32_t fpe_batch_price(const FpeBatchOption* opts, int32_t n, double* prices, int32_t* status,
                         FpeBatchStats* stats);
 
+/** Solver record of one option, summed over its solves (two for in-options). */
+typedef struct FpeSolveInfo {
+    int32_t status, n_solves;
+    int32_t n_accepted, n_rejected, n_lu, n_newton, n_err_solves, n_inner_iters, n_admm_iters, n_refine;
+    int32_t n_sb, n_vb; /* dofs per axis of the option's first solve */
+} FpeSolveInfo;
+
+/** fpe_batch_price with each option's solver record in info[i] (may be NULL)
+ *  and, when qT is not NULL, the terminal state q(T) of every vanilla or out
+ *  option (one solve) in row i of qT (leading dimension ldq, n_sb * n_vb
+ *  values, dof r = I n_vb + i; rows of in-options and of options that did not
+ *  run are left untouched). */
+int32_t fpe_batch_price_detail(const FpeBatchOption* opts, int32_t n, double* prices, int32_t* status,
+                               FpeSolveInfo* info, double* qT, int64_t ldq, FpeBatchStats* stats);
+
 /** Terminal densities q(T) (recombined B-spline coefficients, dof r = I n_vb + i)
- *  of n European options (types 8, 9; strike and type do not enter q(T)) on
- *  their full domains. Row i of qT (leading dimension ldq) receives n_dof[i]
- *  values; status[i] as in fpe_batch_price (FPE_INVALID for other types or
- *  when n_dof exceeds ldq). Returns 0 when the batch ran. */
+ *  of n vanilla (types 8, 9: full domain) or out options (2, 3, 6, 7: the
+ *  barrier-truncated domain); the strike does not enter q(T). Row i of qT
+ *  (leading dimension ldq) receives n_dof[i] values; status[i] as in
+ *  fpe_batch_price (FPE_INVALID for in-types, which need two solves, or when
+ *  n_dof exceeds ldq). Returns 0 when the batch ran. */
 int32_t fpe_batch_terminal_states(const FpeBatchOption* opts, int32_t n, double* qT, int64_t ldq, int32_t* n_dof,
                                   int32_t* status, FpeBatchStats* stats);
+
+/** Mass and stiffness operators M, K of one vanilla or out option's domain as
+ *  assembled on the GPU (before the Radau D-scaling), as 49-point stencils:
+ *  row r = I n_vb + i, entry (J - I + 3) * 7 + (j - i + 3) holds the coupling of
+ *  dof r to dof J n_vb + j (0 outside the grid). M49, K49: cap_dof x 49 doubles.
+ *  Returns 0, FPE_TOO_LARGE for grids that do not fit, else nonzero. */
+int32_t fpe_batch_operators(const FpeBatchOption* opt, int32_t cap_dof, double* M49, double* K49, int32_t* n_sb,
+                            int32_t* n_vb);
 
 /** Pricing operator of one European domain: the s quadrature points s_phys
  *  (n_qs) and the marginal operator marg (n_qs x n_dof, row-major) with

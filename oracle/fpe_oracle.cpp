@@ -1436,6 +1436,9 @@ int orc_price(const OrcSpec* spec, const double* K, int n_K, double* prices, Orc
             stats->n_rejected += s2.n_rejected;
             stats->n_lu += s2.n_lu;
             stats->n_newton += s2.n_newton;
+            stats->n_err_solves += s2.n_err_solves;
+            stats->n_gmres_iters += s2.n_gmres_iters;
+            stats->n_steps_total += s2.n_steps_total;
             stats->success = s1.success && s2.success;
         }
         return 0;

@@ -6,16 +6,17 @@ relative when the Radau trajectory matches, the Newton bound otherwise)."""
 import numpy as np
 import pytest
 
-from test_gpu_parity import HESTON, _close, NEWTON_BOUND, COUNTERS
+from test_gpu_parity import HESTON, _close, _check_option
 
 pytestmark = pytest.mark.gpu
 
 
 def _run(K, **kw):
     from fpe_option_amd import api
-    prices, status, st = api.price_batch(return_stats=True, strike=np.asarray(K, float), **kw)
+    prices, status, st, info = api.price_batch(return_stats=True, return_info=True, strike=np.asarray(K, float),
+                                               **kw)
     assert np.all(status == 0), status
-    return prices, st
+    return prices, st, info
 
 
 @pytest.mark.parametrize("grid", [(16, 8), (40, 20)])
@@ -23,11 +24,10 @@ def test_mixed_matches_oracle(engine, oracle, grid):
     kw = dict(HESTON, disc_mode=1, n_s=grid[0], n_v=grid[1], v_max=1.5, s_max=180.0, ic_sigma_s=0.5,
               ic_sigma_v=0.005, rtol=1e-6, atol=1e-9, option_type=8, complex_mode=1)
     K = [50.0, 60.0, 70.0]
-    g, st = _run(K, precision=1, **kw)
+    g, st, info = _run(K, precision=1, **kw)
     o, ost, rc = oracle.price(K, **kw)
     assert rc == 0
-    same = all(st[c] / len(K) == ost[c] for c in COUNTERS)
-    _close(g, o, rtol=1e-8 if same else NEWTON_BOUND)
+    assert _check_option(f"mixed_{grid[0]}x{grid[1]}", g, o, info[0], ost, gmres=False)
     assert st["n_refine"] > 0
     # rounds per stage solve: FP32 factors of these well-conditioned systems need a few
     n_stage = st["n_newton"] * 2 + st["n_err_solves"]
@@ -37,20 +37,20 @@ def test_mixed_matches_oracle(engine, oracle, grid):
 def test_mixed_reference_mode_barrier(engine, oracle):
     kw = dict(HESTON, n_s=16, n_v=16, num_insert=50, s_max=180.0, option_type=2, barrier=50.0, complex_mode=1)
     K = [55.0, 60.0, 80.0]
-    g, st = _run(K, precision=1, **kw)
-    f, _ = _run(K, precision=0, **kw)
-    o, _, rc = oracle.price(K, **kw)
-    _close(g, o, rtol=NEWTON_BOUND)
-    _close(g, f, rtol=NEWTON_BOUND)
+    g, st, info = _run(K, precision=1, **kw)
+    f, _, _ = _run(K, precision=0, **kw)
+    o, ost, rc = oracle.price(K, **kw)
+    _check_option("mixed_reference_doc", g, o, info[0], ost, gmres=False)
+    _close(g, f)
 
 
 def test_mixed_in_option_parity(engine):
     base = dict(HESTON, disc_mode=1, n_s=24, n_v=12, v_max=1.5, s_max=180.0, ic_sigma_s=0.5, ic_sigma_v=0.005,
                 complex_mode=1, precision=1)
     K = [55.0, 65.0]
-    van, _ = _run(K, option_type=8, **base)
-    doc, _ = _run(K, option_type=2, barrier=50.0, **base)
-    dic, _ = _run(K, option_type=0, barrier=50.0, **base)
+    van, _, _ = _run(K, option_type=8, **base)
+    doc, _, _ = _run(K, option_type=2, barrier=50.0, **base)
+    dic, _, _ = _run(K, option_type=0, barrier=50.0, **base)
     np.testing.assert_allclose(dic + doc, van, rtol=0, atol=1e-13)
 
 
