@@ -109,6 +109,8 @@ SIGNATURES = {
     "fpe_batch_price": ([C.POINTER(FpeBatchOption), _I, _DP, C.POINTER(C.c_int32), C.POINTER(FpeBatchStats)], _I),
     "fpe_batch_price_detail": ([C.POINTER(FpeBatchOption), _I, _DP, C.POINTER(C.c_int32), C.POINTER(FpeSolveInfo),
                                 _DP, C.c_int64, C.POINTER(FpeBatchStats)], _I),
+    "fpe_batch_price_noalloc": ([C.POINTER(FpeBatchOption), _I, _DP, C.POINTER(C.c_int32),
+                                 C.POINTER(FpeBatchStats)], _I),
     "fpe_batch_operators": ([C.POINTER(FpeBatchOption), _I, _DP, _DP, C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
                             _I),
     "fpe_batch_terminal_states": ([C.POINTER(FpeBatchOption), _I, _DP, C.c_int64, C.POINTER(C.c_int32),

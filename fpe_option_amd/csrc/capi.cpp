@@ -473,6 +473,28 @@ int32_t fpe_batch_price(const FpeBatchOption* opts, int32_t n, double* prices, i
     return fpe_batch_price_detail(opts, n, prices, status, nullptr, nullptr, 0, stats);
 }
 
+// fpe_batch_price without device allocation: every device buffer of the call is
+// a slice of the library's fixed scratch array (see include/fpe_engine.h).
+int32_t fpe_batch_price_noalloc(const FpeBatchOption* opts, int32_t n, double* prices, int32_t* status,
+                                FpeBatchStats* stats) {
+    g_last_error.clear();
+    if (n < 0 || (n > 0 && (!opts || !prices || !status))) {
+        g_last_error = "invalid arguments";
+        return 1;
+    }
+    std::vector<fpe::OptionSpec> v(n);
+    for (int i = 0; i < n; ++i) v[i] = from_batch(opts[i]);
+    std::vector<double> p;
+    std::vector<int> s;
+    std::vector<fpe::SolveResult> res;
+    fpe::BatchTimes bt;
+    if (!fpe::price_options(v, p, s, &res, &bt, g_last_error, nullptr, false, true))
+        return bt.scratch_exceeded ? 3 : 2;
+    for (int i = 0; i < n; ++i) prices[i] = p[i], status[i] = s[i];
+    if (stats) fill_stats(res, bt, stats);
+    return 0;
+}
+
 void fpe_compute_basis_1d(FpeCompute* ptr, int32_t axis, int32_t deriv, struct FpeCsrResult* result) {
     *result = FpeCsrResult{nullptr, nullptr, nullptr, 0, 0, 0};
     if (!ptr || axis < 0 || axis > 1) return;

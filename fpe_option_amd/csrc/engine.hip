@@ -48,12 +48,31 @@ struct DBuf {
     T* as() const { return static_cast<T*>(p); }
 };
 
+// Fixed device scratch of the allocation-free entry point (fpe_batch_price_noalloc):
+// every device buffer of such a call is a slice of it. Each GPU the library runs
+// on holds its own copy until the process ends. In host code the name stands for
+// a host-side symbol, not a device address: only the address hipGetSymbolAddress
+// returns (DeviceState::scratch) is passed to kernels and copies.
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
 struct DeviceState {
     bool init = false;
     int n_cu = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[8];
+    unsigned char* scratch = nullptr;  // device address of `scratch` on this device
     DBuf inst, out, kn, qd, bv, bc, fc, st, vc, eg, hs, wm, ow, ws, wi, req, part, queue, order, pdf;
+};
+
+// Device pointers of one batch, from the grow-only buffers or from the scratch.
+struct DevPtrs {
+    Inst* inst = nullptr;
+    InstOut* out = nullptr;
+    double *kn = nullptr, *qd = nullptr, *bv = nullptr, *fc = nullptr, *st = nullptr, *vc = nullptr, *eg = nullptr,
+           *hs = nullptr, *wm = nullptr, *ws = nullptr, *part = nullptr;
+    int *bc = nullptr, *wi = nullptr, *queue = nullptr, *order = nullptr;
+    long long* ow = nullptr;
+    PriceReq* req = nullptr;
 };
 
 std::mutex g_mu;
@@ -75,16 +94,23 @@ bool dev_init(std::string& err) {
                                int(EIG_LDS_S)));
     HIP_OK(hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking));
     for (auto& e : g_dev.ev) HIP_OK(hipEventCreate(&e));
+    // once per device, with the device current: the scratch array's device address
+    HIP_OK(hipGetSymbolAddress(reinterpret_cast<void**>(&g_dev.scratch), HIP_SYMBOL(scratch)));
+    if (!g_dev.scratch) {
+        err = "hipGetSymbolAddress(scratch) returned no address";
+        return false;
+    }
     g_dev.init = true;
     return true;
 }
 
 template <class T>
-bool upload(DBuf& b, const std::vector<T>& h, std::string& err) {
-    if (!b.ensure(h.size() * sizeof(T), err)) return false;
-    if (!h.empty()) HIP_OK(hipMemcpyAsync(b.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, g_dev.stream));
+bool upload(void* dst, const std::vector<T>& h, std::string& err) {
+    if (!h.empty()) HIP_OK(hipMemcpyAsync(dst, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, g_dev.stream));
     return true;
 }
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 }  // namespace
 
@@ -198,8 +224,7 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
     L.stride = (L.stride + 31) & ~31LL;
     L.istride = ((2 * max_nm) + 63) & ~63LL;
     L.has_complex = any_direct;
-    const int n_slots = std::min(n, std::max(1, RADAU_WG_PER_CU * g_dev.n_cu));
-    times.n_slots = n_slots;
+    int n_slots = std::min(n, std::max(1, RADAU_WG_PER_CU * g_dev.n_cu));
     // Radau work queue order: largest estimated factorisation cost (n_sb n_vb^3) first,
     // so the slots finish together instead of waiting on a late large instance.
     std::vector<int> order(n);
@@ -217,23 +242,88 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
         hr[k].out = reqs[k].out;
     }
 
-    // ---- device buffers
+    // ---- device buffers: grow-only allocations, or (RunOptions::noalloc) slices of the
+    // fixed scratch array, with as many Radau slots as the rest of the scratch holds
     DeviceState& G = g_dev;
     hipStream_t st = G.stream;
-    if (!upload(G.inst, hi, err) || !upload(G.kn, kn, err) || !upload(G.qd, qd, err) || !upload(G.ow, ow, err) ||
-        !upload(G.req, hr, err) || !upload(G.order, order, err))
+    const size_t b_inst = sizeof(Inst) * n, b_out = sizeof(InstOut) * n, b_kn = sizeof(double) * kn.size(),
+                 b_qd = sizeof(double) * qd.size(), b_bv = sizeof(double) * o_bv,
+                 b_bc = sizeof(int) * (o_bv / 8 + 1), b_fc = sizeof(double) * o_fc, b_st = sizeof(double) * o_st,
+                 b_vc = sizeof(double) * o_vc, b_eg = sizeof(double) * o_eg,
+                 b_hs = sizeof(double) * std::max(o_hs, 1LL), b_wm = sizeof(double) * o_wm,
+                 b_ow = sizeof(long long) * ow.size(), b_req = sizeof(PriceReq) * std::max<size_t>(hr.size(), 1),
+                 b_part = sizeof(double) * std::max<size_t>(reqs.size(), 1), b_queue = sizeof(int),
+                 b_order = sizeof(int) * order.size();
+    const size_t slot_bytes = sizeof(double) * L.stride + sizeof(int) * L.istride;
+    DevPtrs D;
+    if (opt.noalloc) {
+        if (opt.want_pdf || opt.hist_cap > 0) {
+            err = "the allocation-free path prices only (no PDF or state history)";
+            return false;
+        }
+        const size_t sizes[] = {b_inst, b_out, b_kn, b_qd, b_bv, b_bc, b_fc, b_st, b_vc,
+                                b_eg, b_hs, b_wm, b_ow, b_req, b_part, b_queue, b_order};
+        size_t fixed = 0;
+        for (size_t b : sizes) fixed += align256(b);
+        const size_t cap = NOALLOC_SCRATCH_BYTES;
+        if (fixed + slot_bytes > cap) {
+            times.scratch_exceeded = true;
+            err = "batch needs " + std::to_string(fixed + slot_bytes) + " bytes of device memory with one solver "
+                  "slot; the allocation-free scratch holds " + std::to_string(cap) + " bytes";
+            return false;
+        }
+        n_slots = int(std::min<size_t>(size_t(n_slots), (cap - fixed) / slot_bytes));
+        unsigned char* p = G.scratch;
+        auto take = [&](size_t b) {
+            unsigned char* q = p;
+            p += align256(b);
+            return q;
+        };
+        D.inst = reinterpret_cast<Inst*>(take(b_inst));
+        D.out = reinterpret_cast<InstOut*>(take(b_out));
+        D.kn = reinterpret_cast<double*>(take(b_kn));
+        D.qd = reinterpret_cast<double*>(take(b_qd));
+        D.bv = reinterpret_cast<double*>(take(b_bv));
+        D.bc = reinterpret_cast<int*>(take(b_bc));
+        D.fc = reinterpret_cast<double*>(take(b_fc));
+        D.st = reinterpret_cast<double*>(take(b_st));
+        D.vc = reinterpret_cast<double*>(take(b_vc));
+        D.eg = reinterpret_cast<double*>(take(b_eg));
+        D.hs = reinterpret_cast<double*>(take(b_hs));
+        D.wm = reinterpret_cast<double*>(take(b_wm));
+        D.ow = reinterpret_cast<long long*>(take(b_ow));
+        D.req = reinterpret_cast<PriceReq*>(take(b_req));
+        D.part = reinterpret_cast<double*>(take(b_part));
+        D.queue = reinterpret_cast<int*>(take(b_queue));
+        D.order = reinterpret_cast<int*>(take(b_order));
+        D.ws = reinterpret_cast<double*>(take(sizeof(double) * L.stride * n_slots));
+        D.wi = reinterpret_cast<int*>(take(sizeof(int) * L.istride * n_slots));
+    } else {
+        if (!G.inst.ensure(b_inst, err) || !G.out.ensure(b_out, err) || !G.kn.ensure(b_kn, err) ||
+            !G.qd.ensure(b_qd, err) || !G.bv.ensure(b_bv, err) || !G.bc.ensure(b_bc, err) ||
+            !G.fc.ensure(b_fc, err) || !G.st.ensure(b_st, err) || !G.vc.ensure(b_vc, err) ||
+            !G.eg.ensure(b_eg, err) || !G.hs.ensure(b_hs, err) || !G.wm.ensure(b_wm, err) ||
+            !G.ow.ensure(b_ow, err) || !G.req.ensure(b_req, err) || !G.part.ensure(b_part, err) ||
+            !G.queue.ensure(b_queue, err) || !G.order.ensure(b_order, err) ||
+            !G.ws.ensure(sizeof(double) * L.stride * n_slots, err) ||
+            !G.wi.ensure(sizeof(int) * L.istride * n_slots, err))
+            return false;
+        D.inst = G.inst.as<Inst>(), D.out = G.out.as<InstOut>(), D.kn = G.kn.as<double>();
+        D.qd = G.qd.as<double>(), D.bv = G.bv.as<double>(), D.bc = G.bc.as<int>(), D.fc = G.fc.as<double>();
+        D.st = G.st.as<double>(), D.vc = G.vc.as<double>(), D.eg = G.eg.as<double>(), D.hs = G.hs.as<double>();
+        D.wm = G.wm.as<double>(), D.ow = G.ow.as<long long>(), D.req = G.req.as<PriceReq>();
+        D.part = G.part.as<double>(), D.queue = G.queue.as<int>(), D.order = G.order.as<int>();
+        D.ws = G.ws.as<double>(), D.wi = G.wi.as<int>();
+    }
+    times.n_slots = n_slots;
+    if (!upload(D.inst, hi, err) || !upload(D.kn, kn, err) || !upload(D.qd, qd, err) || !upload(D.ow, ow, err) ||
+        !upload(D.req, hr, err) || !upload(D.order, order, err))
         return false;
-    if (!G.out.ensure(sizeof(InstOut) * n, err) || !G.bv.ensure(sizeof(double) * o_bv, err) ||
-        !G.bc.ensure(sizeof(int) * (o_bv / 8 + 1), err) || !G.fc.ensure(sizeof(double) * o_fc, err) ||
-        !G.st.ensure(sizeof(double) * o_st, err) || !G.vc.ensure(sizeof(double) * o_vc, err) ||
-        !G.eg.ensure(sizeof(double) * o_eg, err) || !G.hs.ensure(sizeof(double) * std::max(o_hs, 1LL), err) ||
-        !G.wm.ensure(sizeof(double) * o_wm, err) || !G.ws.ensure(sizeof(double) * L.stride * n_slots, err) ||
-        !G.wi.ensure(sizeof(int) * L.istride * n_slots, err) ||
-        !G.part.ensure(sizeof(double) * std::max<size_t>(reqs.size(), 1), err) || !G.queue.ensure(sizeof(int), err))
-        return false;
-    HIP_OK(hipMemsetAsync(G.out.p, 0, sizeof(InstOut) * n, st));
-    HIP_OK(hipMemsetAsync(G.queue.p, 0, sizeof(int), st));
-    const Inst* dI = G.inst.as<Inst>();
+    // the scratch keeps the previous call's contents: the counters and the queue
+    // start from zero here (every other buffer is written before it is read)
+    HIP_OK(hipMemsetAsync(D.out, 0, b_out, st));
+    HIP_OK(hipMemsetAsync(D.queue, 0, sizeof(int), st));
+    const Inst* dI = D.inst;
     auto t_setup = std::chrono::steady_clock::now();
     times.setup = std::chrono::duration<double, std::milli>(t_setup - t_start).count();
 
@@ -243,55 +333,55 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
     const bool given = opt.given_qT != nullptr;
     const bool solved = !opt.stop_after_ic;
     HIP_OK(hipEventRecord(G.ev[0], st));
-    hipLaunchKernelGGL(k_basis, dim3(n, 2), dim3(NT), 0, st, dI, G.kn.as<double>(), G.qd.as<double>(),
-                       G.bv.as<double>(), G.bc.as<int>());
+    hipLaunchKernelGGL(k_basis, dim3(n, 2), dim3(NT), 0, st, dI, D.kn, D.qd,
+                       D.bv, D.bc);
     if (!given) {
-        hipLaunchKernelGGL(k_factors, dim3(n, 2), dim3(NT), 0, st, dI, G.qd.as<double>(), G.bv.as<double>(),
-                           G.bc.as<int>(), G.fc.as<double>());
+        hipLaunchKernelGGL(k_factors, dim3(n, 2), dim3(NT), 0, st, dI, D.qd, D.bv,
+                           D.bc, D.fc);
         const int chunks = int(std::min<long long>(64, (max_N + NT - 1) / NT));
-        hipLaunchKernelGGL(k_stencil, dim3(n, chunks), dim3(NT), 0, st, dI, G.fc.as<double>(), G.st.as<double>(),
-                           G.vc.as<double>());
+        hipLaunchKernelGGL(k_stencil, dim3(n, chunks), dim3(NT), 0, st, dI, D.fc, D.st,
+                           D.vc);
     } else {
         for (int s = 0; s < n; ++s) {
             const long long N = (long long)hi[s].nsb * hi[s].nvb;
-            HIP_OK(hipMemcpyAsync(G.vc.as<double>() + hi[s].o_vec + V_QT * N, (*opt.given_qT)[s].data(),
+            HIP_OK(hipMemcpyAsync(D.vc + hi[s].o_vec + V_QT * N, (*opt.given_qT)[s].data(),
                                   sizeof(double) * N, hipMemcpyHostToDevice, st));
         }
     }
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord(G.ev[1], st));
     if (!given) {
-        hipLaunchKernelGGL(k_ic_rhs, dim3(n), dim3(NT), 0, st, dI, G.order.as<int>(), G.qd.as<double>(),
-                           G.bv.as<double>(), G.bc.as<int>(), G.vc.as<double>());
-        hipLaunchKernelGGL(k_eig, dim3(n), dim3(ENT), EIG_LDS_S, st, dI, G.order.as<int>(), 0, EIG_LDS_S,
-                           G.fc.as<double>(), G.eg.as<double>());
-        hipLaunchKernelGGL(k_eig, dim3(n), dim3(ENT), EIG_LDS_V, st, dI, G.order.as<int>(), 1, EIG_LDS_V,
-                           G.fc.as<double>(), G.eg.as<double>());
-        hipLaunchKernelGGL(k_admm, dim3(n), dim3(ANT), ADMM_LDS_MAX, st, dI, G.order.as<int>(), G.st.as<double>(),
-                           G.eg.as<double>(), G.vc.as<double>(), G.out.as<InstOut>());
+        hipLaunchKernelGGL(k_ic_rhs, dim3(n), dim3(NT), 0, st, dI, D.order, D.qd,
+                           D.bv, D.bc, D.vc);
+        hipLaunchKernelGGL(k_eig, dim3(n), dim3(ENT), EIG_LDS_S, st, dI, D.order, 0, EIG_LDS_S,
+                           D.fc, D.eg);
+        hipLaunchKernelGGL(k_eig, dim3(n), dim3(ENT), EIG_LDS_V, st, dI, D.order, 1, EIG_LDS_V,
+                           D.fc, D.eg);
+        hipLaunchKernelGGL(k_admm, dim3(n), dim3(ANT), ADMM_LDS_MAX, st, dI, D.order, D.st,
+                           D.eg, D.vc, D.out);
         HIP_OK(hipGetLastError());
     }
     HIP_OK(hipEventRecord(G.ev[2], st));
     if (solved && !given) {
-        hipLaunchKernelGGL(k_radau, dim3(n_slots), dim3(RNT), 0, st, dI, n, G.queue.as<int>(), G.order.as<int>(),
-                           G.st.as<double>(), G.vc.as<double>(), G.ws.as<double>(), G.wi.as<int>(), L,
-                           G.hs.as<double>(), G.out.as<InstOut>());
+        hipLaunchKernelGGL(k_radau, dim3(n_slots), dim3(RNT), 0, st, dI, n, D.queue, D.order,
+                           D.st, D.vc, D.ws, D.wi, L,
+                           D.hs, D.out);
         HIP_OK(hipGetLastError());
     }
     HIP_OK(hipEventRecord(G.ev[3], st));
     if (solved && !reqs.empty()) {
-        hipLaunchKernelGGL(k_marginal, dim3(n), dim3(NT), 0, st, dI, G.qd.as<double>(), G.bv.as<double>(),
-                           G.bc.as<int>(), G.vc.as<double>(), G.ow.as<long long>(), G.wm.as<double>());
-        hipLaunchKernelGGL(k_price, dim3(int(reqs.size())), dim3(NT), 0, st, dI, G.qd.as<double>(),
-                           G.ow.as<long long>(), G.wm.as<double>(), G.req.as<PriceReq>(), G.part.as<double>());
+        hipLaunchKernelGGL(k_marginal, dim3(n), dim3(NT), 0, st, dI, D.qd, D.bv,
+                           D.bc, D.vc, D.ow, D.wm);
+        hipLaunchKernelGGL(k_price, dim3(int(reqs.size())), dim3(NT), 0, st, dI, D.qd,
+                           D.ow, D.wm, D.req, D.part);
         HIP_OK(hipGetLastError());
     }
     HIP_OK(hipEventRecord(G.ev[4], st));
     std::vector<InstOut> ho(n);
     std::vector<double> parts(reqs.size());
-    HIP_OK(hipMemcpyAsync(ho.data(), G.out.p, sizeof(InstOut) * n, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(ho.data(), D.out, sizeof(InstOut) * n, hipMemcpyDeviceToHost, st));
     if (!reqs.empty() && !opt.stop_after_ic)
-        HIP_OK(hipMemcpyAsync(parts.data(), G.part.p, sizeof(double) * reqs.size(), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipMemcpyAsync(parts.data(), D.part, sizeof(double) * reqs.size(), hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
     float ms = 0;
     HIP_OK(hipEventElapsedTime(&ms, G.ev[0], G.ev[1]));
@@ -321,24 +411,24 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
                 const long long ob = ax == 0 ? hi[s].o_bs : hi[s].o_bv;
                 R.bas[ax].resize(8LL * nq);
                 R.bas_c0[ax].resize(nq);
-                HIP_OK(hipMemcpy(R.bas[ax].data(), G.bv.as<double>() + ob, sizeof(double) * 8 * nq,
+                HIP_OK(hipMemcpy(R.bas[ax].data(), D.bv + ob, sizeof(double) * 8 * nq,
                                  hipMemcpyDeviceToHost));
-                HIP_OK(hipMemcpy(R.bas_c0[ax].data(), G.bc.as<int>() + ob / 8, sizeof(int) * nq,
+                HIP_OK(hipMemcpy(R.bas_c0[ax].data(), D.bc + ob / 8, sizeof(int) * nq,
                                  hipMemcpyDeviceToHost));
             }
         }
         if (opt.want_q0) {
             R.q0.resize(N);
-            HIP_OK(hipMemcpy(R.q0.data(), G.vc.as<double>() + hi[s].o_vec + V_Q0 * N, sizeof(double) * N,
+            HIP_OK(hipMemcpy(R.q0.data(), D.vc + hi[s].o_vec + V_Q0 * N, sizeof(double) * N,
                              hipMemcpyDeviceToHost));
         }
         if (opt.want_stencils && !given) {
             // unscaled M and K as N x 49 rows (entry (dJ + 3) * 7 + dj + 3), from the
             // D-scaled plane-major stencils: M[r][e] = D[r] Ms[r][e] D[col(r, e)]
             std::vector<double> st98(98 * N), Dh(N);
-            HIP_OK(hipMemcpy(st98.data(), G.st.as<double>() + hi[s].o_st, sizeof(double) * 98 * N,
+            HIP_OK(hipMemcpy(st98.data(), D.st + hi[s].o_st, sizeof(double) * 98 * N,
                              hipMemcpyDeviceToHost));
-            HIP_OK(hipMemcpy(Dh.data(), G.vc.as<double>() + hi[s].o_vec + V_D * N, sizeof(double) * N,
+            HIP_OK(hipMemcpy(Dh.data(), D.vc + hi[s].o_vec + V_D * N, sizeof(double) * N,
                              hipMemcpyDeviceToHost));
             const int nvb = hi[s].nvb;
             R.M49.assign(49 * N, 0.0);
@@ -354,22 +444,22 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
         }
         if (opt.want_qT && !opt.stop_after_ic) {
             R.qT.resize(N);
-            HIP_OK(hipMemcpy(R.qT.data(), G.vc.as<double>() + hi[s].o_vec + V_QT * N, sizeof(double) * N,
+            HIP_OK(hipMemcpy(R.qT.data(), D.vc + hi[s].o_vec + V_QT * N, sizeof(double) * N,
                              hipMemcpyDeviceToHost));
         }
         if (opt.hist_cap > 0 && !opt.stop_after_ic) {
             const int nh = std::min(o.n_hist, opt.hist_cap);
             R.hist.assign(nh, std::vector<double>(N));
             for (int k = 0; k < nh; ++k)
-                HIP_OK(hipMemcpy(R.hist[k].data(), G.hs.as<double>() + hi[s].o_hist + k * N, sizeof(double) * N,
+                HIP_OK(hipMemcpy(R.hist[k].data(), D.hs + hi[s].o_hist + k * N, sizeof(double) * N,
                                  hipMemcpyDeviceToHost));
         }
         if (opt.want_pdf && !opt.stop_after_ic) {
             const long long np = (long long)hi[s].nqs * hi[s].nqv;
             if (!G.pdf.ensure(sizeof(double) * np, err)) return false;
             const int blocks = int(std::min<long long>(1024, (np + NT - 1) / NT));
-            hipLaunchKernelGGL(k_pdf, dim3(blocks), dim3(NT), 0, st, dI, s, G.bv.as<double>(), G.bc.as<int>(),
-                               G.vc.as<double>() + hi[s].o_vec + V_QT * N, G.pdf.as<double>());
+            hipLaunchKernelGGL(k_pdf, dim3(blocks), dim3(NT), 0, st, dI, s, D.bv, D.bc,
+                               D.vc + hi[s].o_vec + V_QT * N, G.pdf.as<double>());
             HIP_OK(hipGetLastError());
             R.pdf.resize(np);
             HIP_OK(hipMemcpyAsync(R.pdf.data(), G.pdf.p, sizeof(double) * np, hipMemcpyDeviceToHost, st));
@@ -382,7 +472,7 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
 
 bool price_options(const std::vector<OptionSpec>& opts, std::vector<double>& prices, std::vector<int>& status,
                    std::vector<SolveResult>* results_out, BatchTimes* times_out, std::string& err,
-                   std::vector<std::vector<int>>* used_out, bool want_qT) {
+                   std::vector<std::vector<int>>* used_out, bool want_qT, bool noalloc) {
     const int n = int(opts.size());
     prices.assign(n, 0.0);
     status.assign(n, 0);
@@ -440,7 +530,11 @@ bool price_options(const std::vector<OptionSpec>& opts, std::vector<double>& pri
     BatchTimes bt;
     RunOptions ro;
     ro.want_qT = want_qT;
-    if (!run_batch(solves, doms, reqs, n, ro, prices, res, bt, err)) return false;
+    ro.noalloc = noalloc;
+    if (!run_batch(solves, doms, reqs, n, ro, prices, res, bt, err)) {
+        if (times_out) *times_out = bt;
+        return false;
+    }
     for (int i = 0; i < n; ++i) {
         for (int s : used[i])
             if (res[s].status != 0 && status[i] == 0) status[i] = res[s].status;

@@ -3,6 +3,7 @@
 // kernels over all of them.
 #pragma once
 
+#include <cstddef>
 #include <string>
 #include <vector>
 
@@ -26,7 +27,12 @@ struct SolveResult {
 struct BatchTimes {
     double setup = 0, assemble = 0, ic = 0, radau = 0, payoff = 0, total = 0;
     int n_slots = 0;  // Radau workgroups launched
+    bool scratch_exceeded = false;  // noalloc: the batch does not fit the fixed scratch
 };
+
+// Bytes of the fixed device scratch array that serves every device buffer of an
+// allocation-free call (RunOptions::noalloc, fpe_batch_price_noalloc).
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(256) << 20;
 
 struct Request {
     int solve;         // index into the solve list
@@ -42,6 +48,8 @@ struct RunOptions {
     // when set (one state of N values per solve): no solve runs; prices and PDFs
     // are computed from these terminal states
     const std::vector<std::vector<double>>* given_qT = nullptr;
+    // no device allocation: all buffers are slices of the fixed scratch array (pricing only)
+    bool noalloc = false;
 };
 
 // Largest block-row count the kernels take (the v block width is bounded by MAX_BLOCK = 64).
@@ -62,7 +70,7 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
 // used (when given) receives the indices of each option's solves in results.
 bool price_options(const std::vector<OptionSpec>& opts, std::vector<double>& prices, std::vector<int>& status,
                    std::vector<SolveResult>* results, BatchTimes* times, std::string& err,
-                   std::vector<std::vector<int>>* used = nullptr, bool want_qT = false);
+                   std::vector<std::vector<int>>* used = nullptr, bool want_qT = false, bool noalloc = false);
 
 int device_count();
 

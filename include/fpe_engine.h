@@ -211,6 +211,24 @@ void fpe_batch_option_default(FpeBatchOption* o);
 int32_t fpe_batch_price(const FpeBatchOption* opts, int32_t n, double* prices, int32_t* status,
                         FpeBatchStats* stats);
 
+/** Bytes of the fixed device scratch array that fpe_batch_price_noalloc works in. */
+#define FPE_NOALLOC_SCRATCH_BYTES (256ll << 20)
+
+/** fpe_batch_price without any device allocation. No counterpart in the
+ *  reference: for callers that cannot allocate device memory (code under HIP
+ *  graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ *  Every device buffer of the call (instance data, operators, solver
+ *  workspace) is a slice of a fixed FPE_NOALLOC_SCRATCH_BYTES array that the
+ *  library holds on each GPU it runs on until the process ends; the Radau
+ *  solver runs as many concurrent solves as the rest of the scratch holds.
+ *  Same arguments, prices and statuses as fpe_batch_price. Returns 3 (message
+ *  in fpe_last_error) when the batch does not fit the scratch even with one
+ *  solver slot, 0 on success. Calls share the one array per GPU: the library
+ *  serialises its own calls, and calls that overlap on one GPU (from other
+ *  threads bypassing that lock, or other loaders of the library) would race. */
+int32_t fpe_batch_price_noalloc(const FpeBatchOption* opts, int32_t n, double* prices, int32_t* status,
+                                FpeBatchStats* stats);
+
 /** Solver record of one option, summed over its solves (two for in-options). */
 typedef struct FpeSolveInfo {
     int32_t status, n_solves;
