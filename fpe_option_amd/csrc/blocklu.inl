@@ -47,6 +47,16 @@ struct LuLds {
     unsigned long long tk[4];
 };
 
+// k-step count the MFMA loops of the factorisation are compiled for: the
+// smallest of 5, 8, 10, 15, 16 that covers ceil(m / 4) (column tiles:
+// ceil(TK / 4) of 16). Operand rows m .. 4 TK - 1 of every staged LDS block are
+// kept zero (stage_in, gj_inverse), so padded k-steps add exact zeros and the
+// LDS operand reads need no predicate (rows < 4 TK <= 64, columns < 64: inside
+// the 64 x 65 block).
+__host__ __device__ constexpr int tk_class(int m) {
+    return m <= 20 ? 5 : m <= 32 ? 8 : m <= 40 ? 10 : m <= 60 ? 15 : 16;
+}
+
 // Lane layout of v_mfma_f64_16x16x4f64 (checked on gfx950 with an in-bounds
 // probe): a = A[row0 + lane%16][k0 + lane/16], b = B[k0 + lane/16][col0 + lane%16],
 // acc[j] = C[row0 + lane/16 + 4*j][col0 + lane%16]. (The f32 16x16x4 form places
@@ -228,6 +238,8 @@ __device__ __forceinline__ void apply_panel(LuLds& L, int m, int k0, int nb) {
 }
 
 // Gauss-Jordan inverse of the m x m block in (B[0], B[1]), blocked by panels
+// (a call of its own: its pivot-step chain keeps its values in registers instead
+// of sharing the register budget of the factorisation's unrolled MFMA loops)
 // of GJ_PW pivot steps. Partial pivoting is implicit: step k takes the
 // not-yet-used row p_k with the largest |A[.][k]| (|re| + |im| in complex
 // arithmetic, compared in FP32 to 2^-16 relative, ties to the lower row) and no
@@ -244,7 +256,7 @@ __device__ __forceinline__ void apply_panel(LuLds& L, int m, int k0, int nb) {
 // panel order everything. Sets singular on a zero or non-finite pivot. Ends
 // with a barrier.
 template <bool CPLX, class T>
-__device__ __forceinline__ void gj_inverse(LuLds& L, int m) {
+__device__ __noinline__ void gj_inverse(LuLds& L, int m) {
     m = uniform(m);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const bool lrow = lane < m;
@@ -395,11 +407,13 @@ __device__ __forceinline__ long long stacked_row(long long ps, int ld, int I, in
     return (long long)(I + dJ) * ps + (long long)i * ld;
 }
 
+// Generic forms (any m, runtime k-step count; the operand reads of padded
+// k-steps hit zero rows, column reads stay inside the block).
 // L_JI = A_JI Ainv_I for J = I+1..I+nU, in place. A rows come straight from the
 // panels (a wave takes 16-row strips; their fragments stay in registers), the
 // inverse from LDS (B[0], B[1]).
 template <bool CPLX, class T>
-__device__ void lower_multiply(GAS T* __restrict__ Pr, GAS T* __restrict__ Pi, long long ps, int ld, int I, int m, int nU,
+__device__ void lower_multiply_gen(GAS T* __restrict__ Pr, GAS T* __restrict__ Pi, long long ps, int ld, int I, int m, int nU,
                                const LuLds& L) {
     typedef typename Mf<T>::acc A4;
     constexpr int TG = CPLX ? 2 : 4;  // column tiles per accumulator group
@@ -440,9 +454,8 @@ __device__ void lower_multiply(GAS T* __restrict__ Pr, GAS T* __restrict__ Pi, l
                 for (int s = 0; s < MAX_BLOCK / 4; ++s) {
                     if (s < tk) {
                         const int k = 4 * s + l4;
-                        const bool ok = c < m && k < m;
-                        const T br = ok ? B0[k * LS + c] : T(0);
-                        const T bi = ok ? B1[k * LS + c] : T(0);
+                        const T br = B0[k * LS + c];
+                        const T bi = B1[k * LS + c];
                         p1 = Mf<T>::mfma(ar[s], br, p1);
                         p2 = Mf<T>::mfma(ai[s], bi, p2);
                         p3 = Mf<T>::mfma(ar[s] + ai[s], br + bi, p3);
@@ -472,9 +485,8 @@ __device__ void lower_multiply(GAS T* __restrict__ Pr, GAS T* __restrict__ Pi, l
                     for (int u = 0; u < TG; ++u) {
                         if (ug + u < tn) {
                             const int c = ((ug + u) << 4) + l16;
-                            const bool ok = c < m && k < m;
-                            const T br = ok ? B0[k * LS + c] : T(0);
-                            const T bi = (CPLX && ok) ? B1[k * LS + c] : T(0);
+                            const T br = B0[k * LS + c];
+                            const T bi = CPLX ? B1[k * LS + c] : T(0);
                             mfma_c(accr[u], acci[u], ar[s], ai[s], br, bi, CPLX);
                         }
                     }
@@ -503,7 +515,7 @@ __device__ void lower_multiply(GAS T* __restrict__ Pr, GAS T* __restrict__ Pi, l
 // tiles of a K block run in groups (C tiles loaded up front, one independent
 // accumulator chain per tile).
 template <bool CPLX, class T>
-__device__ void schur_pass(GAS T* __restrict__ Pr, GAS T* __restrict__ Pi, long long ps, int ld, int I, int m, int nU,
+__device__ void schur_pass_gen(GAS T* __restrict__ Pr, GAS T* __restrict__ Pi, long long ps, int ld, int I, int m, int nU,
                            int kb0, int nkb, const LuLds& L) {
     typedef typename Mf<T>::acc A4;
     constexpr int TG = CPLX ? 2 : 4;  // column tiles per accumulator group
@@ -581,9 +593,8 @@ __device__ void schur_pass(GAS T* __restrict__ Pr, GAS T* __restrict__ Pi, long 
                         for (int u = 0; u < TG; ++u) {
                             if (ug + u < tn) {
                                 const int c = ((ug + u) << 4) + l16;
-                                const bool ok = c < m && k < m;
-                                const T br = ok ? Br[k * LS + c] : T(0);
-                                const T bi = (CPLX && ok) ? Bi[k * LS + c] : T(0);
+                                const T br = Br[k * LS + c];
+                                const T bi = CPLX ? Bi[k * LS + c] : T(0);
                                 mfma_c(accr[u], acci[u], ar[s], ai[s], br, bi, CPLX);
                             }
                         }
@@ -607,6 +618,245 @@ __device__ void schur_pass(GAS T* __restrict__ Pr, GAS T* __restrict__ Pi, long 
         if (PF) {
 #pragma unroll
             for (int s2 = 0; s2 < MAX_BLOCK / 4; ++s2) ar[s2] = nr[s2], ai[s2] = ni[s2];
+        } else if (t + RNW < tm) {
+            fetch(t + RNW, ar, ai);
+        }
+    }
+}
+
+// L_JI = A_JI Ainv_I for J = I+1..I+nU, in place. A rows come straight from the
+// panels (a wave takes 16-row strips; their fragments stay in registers), the
+// inverse from LDS (B[0], B[1]). TK k-steps and ceil(TK / 4) column tiles are
+// compile-time (tk_class): the MFMA chains run without branches, LDS operand
+// reads are unpredicated, and global loads read clamped addresses (rows past
+// the stack read its last row, columns past m the block's last column) and
+// select zero, so every load of a strip issues at once.
+template <bool CPLX, class T, int TK>
+__device__ void lower_multiply(GAS T* __restrict__ Pr, GAS T* __restrict__ Pi, long long ps, int ld, int I, int m, int nU,
+                               const LuLds& L) {
+    typedef typename Mf<T>::acc A4;
+    constexpr int TN = (TK + 3) / 4;
+    constexpr int TG = CPLX ? 2 : 4;  // column tiles per accumulator group (real)
+    const T* B0 = lds_block<T>(L, 0);
+    const T* B1 = lds_block<T>(L, 1);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int l16 = lane & 15, l4 = lane >> 4;
+    const int M = nU * m, tm = (M + 15) >> 4;
+    for (int t = wave; t < tm; t += RNW) {
+        const int r0 = t << 4;
+        const int ra = min(r0 + l16, M - 1);  // rows past the stack: its last row (not stored)
+        int dJa;
+        const long long oa = stacked_row(ps, ld, I, m, ra, dJa) + (3 - dJa) * m;
+        T ar[TK], ai[TK];
+#pragma unroll
+        for (int s = 0; s < TK; ++s) {
+            const int k = 4 * s + l4;
+            const int kc = min(k, m - 1);
+            const T vr = Pr[oa + kc];
+            const T vi = CPLX ? Pi[oa + kc] : T(0);
+            ar[s] = k < m ? vr : T(0);
+            ai[s] = (CPLX && k < m) ? vi : T(0);
+        }
+        long long oc[4];
+        bool vc[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int r = r0 + Mf<T>::row(lane, j);
+            vc[j] = r < M;
+            int dJ;
+            oc[j] = vc[j] ? stacked_row(ps, ld, I, m, r, dJ) + (3 - dJ) * m : 0;
+        }
+        if constexpr (CPLX) {
+            // complex products with three real ones per k-step (re = p1 - p2,
+            // im = p3 - p1 - p2): three independent chains per column tile
+#pragma unroll
+            for (int u = 0; u < TN; ++u) {
+                const int c = (u << 4) + l16;
+                A4 p1 = A4{0, 0, 0, 0}, p2 = A4{0, 0, 0, 0}, p3 = A4{0, 0, 0, 0};
+                // operand fragments of k-step s + 1 load while step s's MFMAs run
+                T br = B0[l4 * LS + c], bi = B1[l4 * LS + c];
+#pragma unroll
+                for (int s = 0; s < TK; ++s) {
+                    const int kn = 4 * (s + 1) + l4;
+                    T nbr = T(0), nbi = T(0);
+                    if (s + 1 < TK) nbr = B0[kn * LS + c], nbi = B1[kn * LS + c];
+                    p1 = Mf<T>::mfma(ar[s], br, p1);
+                    p2 = Mf<T>::mfma(ai[s], bi, p2);
+                    p3 = Mf<T>::mfma(ar[s] + ai[s], br + bi, p3);
+                    __builtin_amdgcn_sched_barrier(0);
+                    br = nbr, bi = nbi;
+                }
+                if (c < m) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        if (vc[j]) {
+                            Pr[oc[j] + c] = p1[j] - p2[j];
+                            Pi[oc[j] + c] = p3[j] - p1[j] - p2[j];
+                        }
+                }
+            }
+        } else {
+            // TG column tiles of the strip at a time: independent accumulator chains
+#pragma unroll
+            for (int ug = 0; ug < TN; ug += TG) {
+                A4 acc[TG];
+#pragma unroll
+                for (int u = 0; u < TG; ++u) acc[u] = A4{0, 0, 0, 0};
+                T b[TG], nb[TG];
+#pragma unroll
+                for (int u = 0; u < TG; ++u) b[u] = ug + u < TN ? B0[l4 * LS + ((ug + u) << 4) + l16] : T(0);
+#pragma unroll
+                for (int s = 0; s < TK; ++s) {
+                    const int kn = 4 * (s + 1) + l4;
+#pragma unroll
+                    for (int u = 0; u < TG; ++u)
+                        nb[u] = (s + 1 < TK && ug + u < TN) ? B0[kn * LS + ((ug + u) << 4) + l16] : T(0);
+#pragma unroll
+                    for (int u = 0; u < TG; ++u)
+                        if (ug + u < TN) acc[u] = Mf<T>::mfma(ar[s], b[u], acc[u]);
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int u = 0; u < TG; ++u) b[u] = nb[u];
+                }
+#pragma unroll
+                for (int u = 0; u < TG; ++u) {
+                    const int c = ((ug + u) << 4) + l16;
+                    if (ug + u < TN && c < m) {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            if (vc[j]) Pr[oc[j] + c] = acc[u][j];
+                    }
+                }
+            }
+        }
+    }
+}
+
+// A_JK -= L_JI U_IK for J = I+1..I+nU and K = I+1+kb0 .. I+kb0+nkb, with U_IK
+// staged in LDS (B[0] re, B[1] im in complex arithmetic, one block per pass;
+// B[kb] in real). L strips come from the panels into registers once per strip
+// (real: the next strip of the wave is fetched while the current one's chains
+// run); the column tiles of a K block run in groups (C tiles loaded up front,
+// one independent accumulator chain per tile). Compiled for TK k-steps as
+// lower_multiply: branch-free MFMA chains, unpredicated LDS reads, clamped and
+// selected global loads, predicated stores.
+template <bool CPLX, class T, int TK>
+__device__ void schur_pass(GAS T* __restrict__ Pr, GAS T* __restrict__ Pi, long long ps, int ld, int I, int m, int nU,
+                           int kb0, int nkb, const LuLds& L) {
+    typedef typename Mf<T>::acc A4;
+    constexpr int TN = (TK + 3) / 4;
+    constexpr int TG = CPLX ? 2 : 4;  // column tiles per accumulator group
+    constexpr bool PF = false;        // next-strip prefetch: off (its registers made the inlined
+                                      // Gauss-Jordan chain spill)
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int l16 = lane & 15, l4 = lane >> 4;
+    const int M = nU * m, tm = (M + 15) >> 4;
+    auto fetch = [&](int t, T* fr, T* fi) {
+        const int ra = min((t << 4) + l16, M - 1);
+        int dJa;
+        const long long oa = stacked_row(ps, ld, I, m, ra, dJa) + (3 - dJa) * m;
+#pragma unroll
+        for (int s = 0; s < TK; ++s) {
+            const int k = 4 * s + l4;
+            const int kc = min(k, m - 1);
+            const T vr = Pr[oa + kc];
+            const T vi = CPLX ? Pi[oa + kc] : T(0);
+            fr[s] = k < m ? vr : T(0);
+            fi[s] = (CPLX && k < m) ? vi : T(0);
+        }
+    };
+    T ar[TK], ai[TK], nr[TK], ni[TK];
+#pragma unroll
+    for (int s = 0; s < TK; ++s) ar[s] = ai[s] = nr[s] = ni[s] = T(0);
+    if (wave < tm) fetch(wave, ar, ai);
+    for (int t = wave; t < tm; t += RNW) {
+        const int r0 = t << 4;
+        if (PF && t + RNW < tm) fetch(t + RNW, nr, ni);
+        long long oc[4];
+        int dj[4], fi[4];
+        bool vc[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int r = r0 + Mf<T>::row(lane, j);
+            vc[j] = r < M;
+            dj[j] = 1;
+            oc[j] = stacked_row(ps, ld, I, m, min(r, M - 1), dj[j]);  // rows past the stack: loads only
+            fi[j] = min(r, M - 1) - (dj[j] - 1) * m;  // row within block row I + dj
+        }
+        for (int kb = 0; kb < nkb; ++kb) {
+            const T* Br = lds_block<T>(L, CPLX ? 2 * kb : kb);
+            const T* Bi = lds_block<T>(L, CPLX ? 2 * kb + 1 : kb);
+            const int kk = kb0 + kb;  // K = I + 1 + kk; column block d = kk + 4 - dJ in panel I + dJ
+            // A block's first update (step I = max(J, K) - 3, or 0) finds it as
+            // assembled: zero except within 3 of its diagonal, and only those
+            // entries are written beforehand (the first pass writes the whole
+            // block). Known zeros are selected, not taken from memory.
+            int band[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) band[j] = (I == 0 || dj[j] == 3 || kk == 2) ? fi[j] : -(1 << 20);
+#pragma unroll
+            for (int ug = 0; ug < TN; ug += TG) {
+                T cr[TG][4], ci[TG][4];
+                A4 accr[TG], acci[TG];
+#pragma unroll
+                for (int u = 0; u < TG; ++u) {
+                    const int c = ((ug + u) << 4) + l16;
+                    const int cc = min(c, m - 1);
+                    accr[u] = A4{0, 0, 0, 0}, acci[u] = A4{0, 0, 0, 0};
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const bool ok = ug + u < TN && vc[j] && c < m &&
+                                        (band[j] < 0 || (c - band[j] <= 3 && band[j] - c <= 3));
+                        const long long o = oc[j] + (long long)(kk + 4 - dj[j]) * m + cc;
+                        // known zeros (outside the band on a first update) are not loaded
+                        cr[u][j] = ok ? Pr[o] : T(0);
+                        ci[u][j] = (CPLX && ok) ? Pi[o] : T(0);
+                    }
+                }
+                // operand fragments of k-step s + 1 load while step s's MFMAs run
+                T br[TG], bi[TG], nbr[TG], nbi[TG];
+#pragma unroll
+                for (int u = 0; u < TG; ++u) {
+                    const int c = ((ug + u) << 4) + l16;
+                    br[u] = ug + u < TN ? Br[l4 * LS + c] : T(0);
+                    bi[u] = (CPLX && ug + u < TN) ? Bi[l4 * LS + c] : T(0);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int s = 0; s < TK; ++s) {
+                    const int kn = 4 * (s + 1) + l4;
+#pragma unroll
+                    for (int u = 0; u < TG; ++u) {
+                        const int c = ((ug + u) << 4) + l16;
+                        const bool ld = s + 1 < TK && ug + u < TN;
+                        nbr[u] = ld ? Br[kn * LS + c] : T(0);
+                        nbi[u] = (CPLX && ld) ? Bi[kn * LS + c] : T(0);
+                    }
+#pragma unroll
+                    for (int u = 0; u < TG; ++u)
+                        if (ug + u < TN) mfma_c(accr[u], acci[u], ar[s], ai[s], br[u], bi[u], CPLX);
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int u = 0; u < TG; ++u) br[u] = nbr[u], bi[u] = nbi[u];
+                }
+#pragma unroll
+                for (int u = 0; u < TG; ++u) {
+                    const int c = ((ug + u) << 4) + l16;
+                    if (ug + u < TN && c < m) {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            if (vc[j]) {
+                                const long long o = oc[j] + (long long)(kk + 4 - dj[j]) * m + c;
+                                Pr[o] = cr[u][j] - accr[u][j];
+                                if (CPLX) Pi[o] = ci[u][j] - acci[u][j];
+                            }
+                    }
+                }
+            }
+        }
+        if (PF) {
+#pragma unroll
+            for (int s2 = 0; s2 < TK; ++s2) ar[s2] = nr[s2], ai[s2] = ni[s2];
         } else if (t + RNW < tm) {
             fetch(t + RNW, ar, ai);
         }
@@ -706,10 +956,13 @@ __device__ void assemble_E(const double* __restrict__ Ms, const double* __restri
 // block). Padding columns of the staged rows are written with zeros.
 // H = 1 issues the loads of all rows before any store (one memory latency,
 // twice the registers).
+// Rows m .. mz - 1 of the LDS block are written with zeros (the padded operand
+// rows of the TK-compiled MFMA loops; mz = 4 tk_class(m)).
 template <bool CPLX, class T, int H = 2>
 __device__ __forceinline__ void stage_in(T* dr, T* di, GAS const T* sr, GAS const T* si, int ld, int m) {
     constexpr int R = MAX_BLOCK / RNW / H;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int mz = max(m, 4 * tk_class(m));
 #pragma unroll
     for (int h = 0; h < H; ++h) {
         T vr[R], vi[R];
@@ -728,7 +981,7 @@ __device__ __forceinline__ void stage_in(T* dr, T* di, GAS const T* sr, GAS cons
 #pragma unroll
         for (int u = 0; u < R; ++u) {
             const int r = w + RNW * (u + R * h);
-            if (r < m) {
+            if (r < mz) {  // rows m..mz-1: vr, vi hold zeros
                 dr[r * LS + lane] = vr[u];
                 if (CPLX) di[r * LS + lane] = vi[u];
             }
@@ -777,9 +1030,9 @@ __device__ __noinline__ void pivot_block_out(LuLds& L, T* sr, T* si, int ld, int
                        ld, m);
 }
 
-// Factor the panels in place. Returns nonzero if a diagonal block was singular.
+// Factor the panels in place (generic loops). Returns nonzero if a diagonal block was singular.
 template <bool CPLX, class T>
-__device__ __noinline__ int block_lu(T* __restrict__ Pr_, T* __restrict__ Pi_, int ns, int m, LuLds& L) {
+__device__ __noinline__ int block_lu_gen(T* __restrict__ Pr_, T* __restrict__ Pi_, int ns, int m, LuLds& L) {
     GAS T* const Pr = uniform(gas(Pr_));
     GAS T* const Pi = uniform(gas(Pi_));
     ns = uniform(ns), m = uniform(m);
@@ -810,7 +1063,7 @@ __device__ __noinline__ int block_lu(T* __restrict__ Pr_, T* __restrict__ Pi_, i
         pivot_block_out<CPLX, T>(L, (T*)(PIr + 3 * m), CPLX ? (T*)(PIi + 3 * m) : nullptr, ld, m);
         const int nU = (ns - 1 - I) < 3 ? (ns - 1 - I) : 3;
         if (nU == 0) break;
-        lower_multiply<CPLX, T>(Pr, Pi, ps, ld, I, m, nU, L);
+        lower_multiply_gen<CPLX, T>(Pr, Pi, ps, ld, I, m, nU, L);
         __syncthreads();
         lap(2);
         for (int kb0 = 0; kb0 < nU; kb0 += KPASS) {
@@ -823,13 +1076,88 @@ __device__ __noinline__ int block_lu(T* __restrict__ Pr_, T* __restrict__ Pi_, i
             }
             __syncthreads();
             lap(1);
-            schur_pass<CPLX, T>(Pr, Pi, ps, ld, I, m, nU, kb0, nkb, L);
+            schur_pass_gen<CPLX, T>(Pr, Pi, ps, ld, I, m, nU, kb0, nkb, L);
             __syncthreads();
             lap(3);
         }
     }
     __syncthreads();
     return L.singular;
+}
+
+// Factor the panels in place. Returns nonzero if a diagonal block was singular.
+// TK = tk_class(m): the k-steps the L multiply and Schur update are compiled for.
+template <bool CPLX, class T, int TK>
+__device__ __noinline__ int block_lu_tk(T* __restrict__ Pr_, T* __restrict__ Pi_, int ns, int m, LuLds& L) {
+    GAS T* const Pr = uniform(gas(Pr_));
+    GAS T* const Pi = uniform(gas(Pi_));
+    ns = uniform(ns), m = uniform(m);
+    const int ld = 7 * m;
+    const long long ps = (long long)m * ld;
+    const int tid = threadIdx.x;
+    if (tid == 0) L.singular = 0;
+    unsigned long long mark = wall_clock64();
+    auto lap = [&](int k) {
+        if (tid == 0) {
+            const unsigned long long now = wall_clock64();
+            L.tk[k] += now - mark;
+            mark = now;
+        }
+    };
+    // U blocks per Schur pass: 2 fit the staging buffers in real arithmetic, 1 in complex
+    constexpr int KPASS = CPLX ? 1 : 2;
+    T* B0 = lds_block<T>(L, 0);
+    T* B1 = lds_block<T>(L, 1);
+    for (int I = 0; I < ns; ++I) {
+        GAS T* PIr = Pr + I * ps;
+        GAS T* PIi = Pi + I * ps;
+        pivot_block_in<CPLX, T>(L, (const T*)(PIr + 3 * m), CPLX ? (const T*)(PIi + 3 * m) : nullptr, ld, m);
+        __syncthreads();
+        lap(1);
+        gj_inverse<CPLX, T>(L, m);
+        lap(0);
+        pivot_block_out<CPLX, T>(L, (T*)(PIr + 3 * m), CPLX ? (T*)(PIi + 3 * m) : nullptr, ld, m);
+        const int nU = (ns - 1 - I) < 3 ? (ns - 1 - I) : 3;
+        if (nU == 0) break;
+        lower_multiply<CPLX, T, TK>(Pr, Pi, ps, ld, I, m, nU, L);
+        __syncthreads();
+        lap(2);
+        for (int kb0 = 0; kb0 < nU; kb0 += KPASS) {
+            const int nkb = (nU - kb0) < KPASS ? (nU - kb0) : KPASS;
+            for (int kb = 0; kb < nkb; ++kb) {
+                T* Br = lds_block<T>(L, CPLX ? 2 * kb : kb);
+                T* Bi = CPLX ? lds_block<T>(L, 2 * kb + 1) : nullptr;
+                const int col = (4 + kb0 + kb) * m;
+                stage_in<CPLX, T, 1>(Br, Bi, PIr + col, CPLX ? PIi + col : nullptr, ld, m);
+            }
+            __syncthreads();
+            lap(1);
+            schur_pass<CPLX, T, TK>(Pr, Pi, ps, ld, I, m, nU, kb0, nkb, L);
+            __syncthreads();
+            lap(3);
+        }
+    }
+    __syncthreads();
+    return L.singular;
+}
+
+// Factorisation code paths: LU_GENERIC (runtime k-step loops, one code path for
+// every m) or LU_COMPILED_K (MFMA loops compiled for tk_class(m) k-steps). The
+// per-grid choice comes from the tuning table (fpe_option_amd/tuning_gfx950.json).
+enum { LU_GENERIC = 0, LU_COMPILED_K = 1 };
+
+// One call site per use in the solver kernel (the code path is chosen inside).
+template <bool CPLX, class T>
+__device__ __noinline__ int block_lu(T* __restrict__ Pr, T* __restrict__ Pi, int ns, int m, int variant, LuLds& L) {
+    m = uniform(m), variant = uniform(variant);
+    if (variant == LU_GENERIC) return block_lu_gen<CPLX, T>(Pr, Pi, ns, m, L);
+    switch (tk_class(m)) {
+        case 5: return block_lu_tk<CPLX, T, 5>(Pr, Pi, ns, m, L);
+        case 8: return block_lu_tk<CPLX, T, 8>(Pr, Pi, ns, m, L);
+        case 10: return block_lu_tk<CPLX, T, 10>(Pr, Pi, ns, m, L);
+        case 15: return block_lu_tk<CPLX, T, 15>(Pr, Pi, ns, m, L);
+        default: return block_lu_tk<CPLX, T, 16>(Pr, Pi, ns, m, L);
+    }
 }
 
 // Single right-hand side solve with the block factors, latency-shaped: each

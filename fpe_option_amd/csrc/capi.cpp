@@ -31,6 +31,7 @@ fpe::OptionSpec from_batch(const FpeBatchOption& b) {
     o.atol = b.atol > 0 ? b.atol : 1e-6;
     o.knot_width_s = b.knot_width_s, o.knot_width_v = b.knot_width_v;
     o.knot_center_v = b.knot_center_v;
+    o.lu_variant = b.lu_variant, o.max_steps = b.max_steps, o.refine_max = b.refine_max;
     return o;
 }
 

@@ -44,6 +44,8 @@ struct Inst {
     int nsk, nvk;             // knot counts
     int s_lneu, s_rneu;       // s-boundary: 1 = Neumann (merge), 0 = Dirichlet (drop)
     int cmode, prec, exact, hist_cap;
+    int lu_var;               // factorisation code path (LU_GENERIC / LU_COMPILED_K)
+    int max_steps, refine_max;  // Radau step limit; mixed-precision refinement rounds per stage solve
     long long o_sk, o_vk;     // knots (array KN)
     long long o_sq, o_vq;     // s: x, w, phys (3*nqs); v: x, w (2*nqv)   (array QD)
     long long o_bs, o_bv;     // basis per point: 8 doubles (BV) and first column (BC)

@@ -209,6 +209,7 @@ SolveSpec solve_for(const OptionSpec& o, int t) {
     s.complex_mode = o.complex_mode, s.precision = o.precision, s.exact_constants = o.exact_constants;
     s.knot_width_s = o.knot_width_s, s.knot_width_v = o.knot_width_v;
     s.knot_center_v = o.knot_center_v;
+    s.lu_variant = o.lu_variant, s.max_steps = o.max_steps, s.refine_max = o.refine_max;
     return s;
 }
 

@@ -23,6 +23,9 @@ struct OptionSpec {
     double ic_sigma_s, ic_sigma_v, rtol, atol;
     double knot_width_s, knot_width_v;
     double knot_center_v;  // span mode: v breakpoints cluster here (<= 0: at V0)
+    int lu_variant;        // factorisation code path: 0 tuned, 1 generic, 2 compiled k-steps
+    int max_steps;         // Radau step limit (<= 0: 100000, the reference's)
+    int refine_max;        // mixed precision: refinement rounds per stage solve (0: 10, < 0: none)
 };
 
 // One Fokker-Planck solve on a (possibly barrier-truncated) domain.
@@ -35,6 +38,7 @@ struct SolveSpec {
     int complex_mode, precision, exact_constants;
     double knot_width_s, knot_width_v;
     double knot_center_v;
+    int lu_variant, max_steps, refine_max;
 };
 
 // Knots and quadrature of one solve.

@@ -1,9 +1,12 @@
 // Host orchestration of the batched GPU pricing path.
 #include <hip/hip_runtime.h>
 
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <mutex>
 
@@ -78,6 +81,61 @@ struct DevPtrs {
 std::mutex g_mu;
 DeviceState g_dev;
 
+// Tuning table: measured factorisation code path per grid shape (n_sb, n_vb),
+// read once from tuning_gfx950.txt next to this library (written by
+// fpe_option_amd/autotune.py from GPU timings). Lines: "n_sb n_vb variant".
+struct TuneRow {
+    int nsb, nvb, variant;
+};
+std::vector<TuneRow> g_tune;
+bool g_tune_loaded = false;
+
+void load_tuning() {
+    if (g_tune_loaded) return;
+    g_tune_loaded = true;
+    Dl_info info;
+    if (!dladdr(reinterpret_cast<void*>(&load_tuning), &info) || !info.dli_fname) return;
+    std::string path(info.dli_fname);
+    const size_t slash = path.rfind('/');
+    path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/tuning_gfx950.txt";
+    FILE* f = std::fopen(path.c_str(), "r");
+    if (!f) return;
+    char line[256];
+    while (std::fgets(line, sizeof line, f)) {
+        TuneRow r;
+        if (line[0] == '#') continue;
+        if (std::sscanf(line, "%d %d %d", &r.nsb, &r.nvb, &r.variant) == 3 && (r.variant == 0 || r.variant == 1))
+            g_tune.push_back(r);
+    }
+    std::fclose(f);
+}
+
+// The tuned code path of a grid: the measured shape with the nearest block
+// width at or above n_vb (else the widest), then the nearest block-row count.
+int tuned_variant(int nsb, int nvb) {
+    load_tuning();
+    int best = -1;
+    for (size_t k = 0; k < g_tune.size(); ++k) {
+        const TuneRow& r = g_tune[k];
+        if (best < 0) {
+            best = int(k);
+            continue;
+        }
+        const TuneRow& b = g_tune[best];
+        const bool r_cov = r.nvb >= nvb, b_cov = b.nvb >= nvb;
+        if (r_cov != b_cov) {
+            if (r_cov) best = int(k);
+            continue;
+        }
+        if (r.nvb != b.nvb) {
+            if (r_cov ? r.nvb < b.nvb : r.nvb > b.nvb) best = int(k);
+            continue;
+        }
+        if (std::abs(r.nsb - nsb) < std::abs(b.nsb - nsb)) best = int(k);
+    }
+    return best < 0 ? LU_GENERIC : g_tune[best].variant;
+}
+
 bool dev_init(std::string& err) {
     if (g_dev.init) return true;
     int n = 0;
@@ -139,11 +197,11 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
                BatchTimes& times, std::string& err) {
     std::lock_guard<std::mutex> lk(g_mu);
     auto t_start = std::chrono::steady_clock::now();
-    if (!dev_init(err)) return false;
     const int n = int(solves.size());
     results.assign(n, SolveResult());
     out_prices.assign(n_out, 0.0);
-    if (n == 0) return true;
+    if (n == 0) return true;  // nothing to run (every option was screened out on the host)
+    if (!dev_init(err)) return false;
 
     // ---- pack host arrays
     std::vector<Inst> hi(n);
@@ -168,6 +226,9 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
         }
         I.s_lneu = sp.s_left_neumann, I.s_rneu = sp.s_right_neumann;
         I.cmode = sp.complex_mode, I.prec = sp.precision, I.exact = sp.exact_constants;
+        I.lu_var = sp.lu_variant == 1 ? LU_GENERIC : sp.lu_variant == 2 ? LU_COMPILED_K : tuned_variant(I.nsb, I.nvb);
+        I.max_steps = sp.max_steps > 0 ? sp.max_steps : 100000;  // radau.mojo:257
+        I.refine_max = sp.refine_max == 0 ? 10 : (sp.refine_max < 0 ? 0 : sp.refine_max);
         I.hist_cap = opt.hist_cap;
         I.o_sk = (long long)kn.size();
         kn.insert(kn.end(), d.s_knots.begin(), d.s_knots.end());

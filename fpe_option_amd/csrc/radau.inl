@@ -287,7 +287,7 @@ __global__ void __launch_bounds__(RNT, RADAU_WG_PER_CU * RNW / 4) k_radau(const 
         // the stencils) drive corrections until |r|_inf <= 1e-12 |b|_inf (at most
         // 10 rounds), so the result is the FP64 path's solve up to that residual.
         double* ring = S.lu.B[0];  // the factorisation's staging LDS is idle while solving
-        constexpr int REFINE_MAX = 10;
+        const int REFINE_MAX = in.refine_max;  // 10 unless the caller sets another limit
         constexpr double REFINE_TOL = 1e-12;
         int n_ref = 0;
         bool refine_fail = false;  // a stage solve left its refinement above REFINE_TOL
@@ -352,7 +352,7 @@ __global__ void __launch_bounds__(RNT, RADAU_WG_PER_CU * RNW / 4) k_radau(const 
         };
 
         while (posneg * (t1 - t) > uround * fmax(fabs(t), fabs(t1))) {
-            if (n_steps > 100000 || ++guard > 2000000) { status = ST_MAX_STEPS; break; }
+            if (n_steps > in.max_steps || ++guard > 20LL * in.max_steps) { status = ST_MAX_STEPS; break; }
             if (posneg * (t + 1.01 * h - t1) > 0.0) h = t1 - t;
             if (!(fabs(h) >= 1e-14)) { status = ST_H_UNDERFLOW; break; }
             bool need_lu = (h_lu == 0.0);
@@ -367,24 +367,24 @@ __global__ void __launch_bounds__(RNT, RADAU_WG_PER_CU * RNW / 4) k_radau(const 
                     assemble_E<false, float>(Ms, Ks, ns, m, U1, 0.0, h, Pf, (float*)nullptr);
                     __syncthreads();
                     lap(tk_asm);
-                    bad = block_lu<false, float>(Pf, (float*)nullptr, ns, m, S.lu);
+                    bad = block_lu<false, float>(Pf, (float*)nullptr, ns, m, in.lu_var, S.lu);
                     lap(tk_lu);
                     assemble_E<true, float>(Ms, Ks, ns, m, ALPH, BETA, h, Crf, Cif);
                     __syncthreads();
                     lap(tk_asm);
-                    bad |= block_lu<true, float>(Crf, Cif, ns, m, S.lu);
+                    bad |= block_lu<true, float>(Crf, Cif, ns, m, in.lu_var, S.lu);
                     lap(tk_lu);
                 } else {
                     assemble_E<false, double>(Ms, Ks, ns, m, U1, 0.0, h, P, (double*)nullptr);
                     __syncthreads();
                     lap(tk_asm);
-                    bad = block_lu<false, double>(P, (double*)nullptr, ns, m, S.lu);
+                    bad = block_lu<false, double>(P, (double*)nullptr, ns, m, in.lu_var, S.lu);
                     lap(tk_lu);
                     if (direct) {
                         assemble_E<true, double>(Ms, Ks, ns, m, ALPH, BETA, h, Cr, Ci);
                         __syncthreads();
                         lap(tk_asm);
-                        bad |= block_lu<true, double>(Cr, Ci, ns, m, S.lu);
+                        bad |= block_lu<true, double>(Cr, Ci, ns, m, in.lu_var, S.lu);
                         lap(tk_lu);
                     }
                 }

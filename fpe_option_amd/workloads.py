@@ -3,8 +3,9 @@
 The 4096-option batch: Heston parameters drawn uniformly from
 kappa in [0.5, 4], theta in [0.02, 0.2], eta in [0.1, 0.8], rho in [-0.9, 0],
 V0 in [0.02, 0.3], r in [0, 0.1], T in [0.1, 2], S0 = 60, K in [45, 75];
-grids (n_s, n_v) index-paired from {40, 60, 80, 120} x {20, 30, 40, 60}
-(aspect 2:1); 50 % European calls/puts, 50 % single-barrier out/in options
+grids (n_s, n_v) with n_s and n_v drawn independently from {40, 60, 80, 120}
+and {20, 30, 40, 60} (all 16 shapes; pairing="paired" draws one index for
+both, the 4 shapes of aspect 2:1); 50 % European calls/puts, 50 % single-barrier out/in options
 over {down, up} x {out, in} x {call, put} with B = S0 (1 -+ u), u in [0.10, 0.25].
 Span discretisation: cubic B-splines with sinh-clustered knots, 4-point
 Gauss-Legendre per span, s in [0, 180], v in [0, 1.5], initial Gaussian
@@ -18,7 +19,7 @@ BARRIER_TYPES = np.array([2, 3, 6, 7, 0, 1, 4, 5])  # DO call/put, UO call/put, 
 
 
 def batch_columns(n=4096, seed=SEED, rtol=1e-4, atol=1e-6, complex_mode=1, precision=0, grid=None,
-                  exact_constants=0):
+                  exact_constants=0, pairing="independent"):
     rng = np.random.Generator(np.random.PCG64(seed))
     cols = dict(kappa=rng.uniform(0.5, 4.0, n), theta=rng.uniform(0.02, 0.2, n), sigma=rng.uniform(0.1, 0.8, n),
                 rho=rng.uniform(-0.9, 0.0, n), V0=rng.uniform(0.02, 0.3, n), r=rng.uniform(0.0, 0.1, n),
@@ -27,8 +28,9 @@ def batch_columns(n=4096, seed=SEED, rtol=1e-4, atol=1e-6, complex_mode=1, preci
     cols["S0"] = np.full(n, S0)
     cols["strike"] = rng.uniform(45.0, 75.0, n)
     k = rng.integers(0, 4, n)
+    kv = k if pairing == "paired" else rng.integers(0, 4, n)
     cols["n_s"] = GRIDS[k, 0] if grid is None else np.full(n, grid[0])
-    cols["n_v"] = GRIDS[k, 1] if grid is None else np.full(n, grid[1])
+    cols["n_v"] = GRIDS[kv, 1] if grid is None else np.full(n, grid[1])
     euro = rng.uniform(0.0, 1.0, n) < 0.5
     call = rng.uniform(0.0, 1.0, n) < 0.5
     bt = BARRIER_TYPES[rng.integers(0, 8, n)]

@@ -162,6 +162,11 @@ typedef struct FpeBatchOption {
     double rtol, atol;             /* <= 0 -> 1e-4 / 1e-6 */
     double knot_width_s, knot_width_v; /* span mode, <= 0 -> 0.1 */
     double knot_center_v;              /* span mode: v breakpoints cluster here; <= 0 -> V0 */
+    int32_t lu_variant; /* factorisation code path: 0 tuned per grid (tuning_gfx950.txt), 1 generic loops,
+                           2 MFMA loops compiled for the block width's k-steps */
+    int32_t max_steps;  /* Radau step limit; <= 0 -> 100000 (the reference's) */
+    int32_t refine_max; /* FPE_PREC_MIXED refinement rounds per stage solve; 0 -> 10, < 0 -> none */
+    int32_t reserved;
 } FpeBatchOption;
 
 /** Per-batch counters (summed over the solves of the batch). */

@@ -65,6 +65,19 @@ def test_create_rejects_invalid_parameters(args):
     assert not h
 
 
+def test_host_screened_statuses_without_a_gpu():
+    """Invalid parameters (FPE_INVALID, 1) and grids wider than the kernels take
+    (FPE_TOO_LARGE, 5) are decided on the host: a batch of only such options
+    completes without touching the device, price 0 each."""
+    from fpe_option_amd import api
+    kw = dict(HESTON, kappa=[1.2, -1.0, 1.2])
+    prices, status = api.price_batch(**kw, strike=60.0, option_type=[8, 8, 2], n_s=[16, 16, 16],
+                                     n_v=[100, 8, 8], barrier=[0.0, 0.0, 70.0], disc_mode=1, v_max=1.5,
+                                     s_max=180.0)
+    assert list(status) == [5, 1, 1]
+    assert np.all(prices == 0.0)
+
+
 def test_null_handles_give_empty_results():
     lib = L.lib()
     r = L.FpeVecResult()

@@ -27,13 +27,13 @@ NEWTON_BOUND = 1e-5
 KEYS = ("n_accepted", "n_rejected", "n_lu", "n_newton", "n_err_solves")
 
 
-def _run(cases, precision=0):
+def _run(cases, precision=0, lu_variant=0):
     """One GPU batch: every case at every fixture strike."""
     from fpe_option_amd import api
     rows = []
     for name, c in cases.items():
         for j, K in enumerate(c["strikes"]):
-            rows.append((name, j, dict(c["kw"], strike=float(K), precision=precision)))
+            rows.append((name, j, dict(c["kw"], strike=float(K), precision=precision, lu_variant=lu_variant)))
     cols = {k: np.array([r[2][k] for r in rows]) for k in rows[0][2]}
     g, status, info, qT = api.price_batch(return_info=True, return_qT=True, **cols)
     return rows, g, status, info, qT
@@ -74,14 +74,17 @@ def test_config2_against_fixture(engine, precision):
     _check(rows, g, status, info, qT, cases, f"config2_prec{precision}")
 
 
-@pytest.mark.parametrize("mode", ["direct", "mixed", "gmres"])
-def test_grid_shapes_against_fixture(engine, mode):
+@pytest.mark.parametrize("mode,lu_variant", [("direct", 0), ("mixed", 0), ("gmres", 0), ("direct", 1), ("direct", 2),
+                                             ("mixed", 1), ("mixed", 2)])
+def test_grid_shapes_against_fixture(engine, mode, lu_variant):
+    """lu_variant 0: the tuned factorisation code path of each shape; 1 and 2
+    force the generic loops and the loops compiled for the block width."""
     allc = golden_data.load("grids")
     suffix = "_gmres" if mode == "gmres" else "_direct"
     cases = {k: v for k, v in allc.items() if k.endswith(suffix)}
     assert len(cases) == (4 if mode == "gmres" else 16)
-    rows, g, status, info, qT = _run(cases, 1 if mode == "mixed" else 0)
-    _check(rows, g, status, info, qT, cases, f"grids_{mode}")
+    rows, g, status, info, qT = _run(cases, 1 if mode == "mixed" else 0, lu_variant)
+    _check(rows, g, status, info, qT, cases, f"grids_{mode}_lu{lu_variant}")
 
 
 def test_operators_against_fixture(engine):

@@ -360,6 +360,21 @@ def test_oversized_grid_gets_status_and_rest_runs(engine, oracle):
         fe.price(**HESTON, n_s=38, n_v=200, K=60.0)
 
 
+def test_statuses_forced_from_the_host(engine, oracle):
+    """A step limit below what the solve needs gives FPE_MAX_STEPS (2) and, on
+    the reference-shaped surface, an error with the reason; mixed precision with
+    no refinement rounds reports FPE_REFINE_FAIL (7) and still returns the price
+    of the solve it ran (the FP32 factors leave Newton to correct it)."""
+    from fpe_option_amd import api
+    kw = dict(HESTON, disc_mode=1, n_s=16, n_v=8, v_max=1.5, s_max=180.0, ic_sigma_s=0.5, ic_sigma_v=0.005,
+              option_type=8, complex_mode=1, strike=60.0)
+    p, st = api.price_batch(**kw, max_steps=[0, 3])
+    assert list(st) == [0, 2]
+    p2, st2 = api.price_batch(**kw, precision=1, refine_max=[0, -1])
+    assert list(st2) == [0, 7]
+    assert abs(p2[1] - p[0]) < 1e-2 * p[0] and abs(p2[0] - p[0]) < 1e-8 * p[0]
+
+
 def test_native_module_drives_reference_shaped_wrapper(engine, oracle):
     # the dict/array shapes the reference's pricer.Compute converts (pricer.py:124-197)
     import scipy.sparse as sp
