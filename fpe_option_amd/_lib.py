@@ -112,6 +112,7 @@ SIGNATURES = {
                                 _DP, C.c_int64, C.POINTER(FpeBatchStats)], _I),
     "fpe_batch_price_noalloc": ([C.POINTER(FpeBatchOption), _I, _DP, C.POINTER(C.c_int32),
                                  C.POINTER(FpeBatchStats)], _I),
+    "fpe_batch_apply_operators": ([C.POINTER(FpeBatchOption), _I, _DP, _DP, _DP], _I),
     "fpe_batch_operators": ([C.POINTER(FpeBatchOption), _I, _DP, _DP, C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
                             _I),
     "fpe_batch_terminal_states": ([C.POINTER(FpeBatchOption), _I, _DP, C.c_int64, C.POINTER(C.c_int32),
@@ -120,6 +121,8 @@ SIGNATURES = {
                                        C.POINTER(C.c_int32)], _I),
     "fpe_mlp_forward": ([_P, _I, _I, _P, _P, _I, _I, _P, _P, _P], _I),
     "fpe_mlp_backward": ([_P, _P, _P, _I, _I, _I, _P, _P, _P, _P, _P], _I),
+    "fpe_mlp_forward_f32": ([_P, _I, _I, _P, _P, _I, _I, _P, _P, _P], _I),
+    "fpe_mlp_backward_f32": ([_P, _P, _P, _I, _I, _I, _P, _P, _P, _P, _P], _I),
     "fpe_mlp_adam": ([_P, _P, _P, _P, _P, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float, _I, _P], _I),
     "fpe_mlp_mse": ([_P, _P, _P, _I, _I, _P, _P, _P], _I),
     "fpe_mlp_to_bf16": ([_P, _P, C.c_int64, _P], _I),
@@ -130,6 +133,8 @@ SIGNATURES = {
     "fpe_device_download": ([_P, _P, C.c_int64], _I),
     "fpe_device_synchronize": ([], _I),
     "fpe_surrogate_price": ([_P, _I, _I, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P], _I),
+    "fpe_pod_fit": ([_P, _I, _I, _P, _I, _I, _P, _P, _P, _P, _P, C.POINTER(C.c_int32), _P], _I),
+    "fpe_svd_jacobi": ([_P, _I, _I, _I, _P, _P, C.POINTER(C.c_int32), _P], _I),
     "fpe_last_error": ([], C.c_char_p),
     "fpe_device_count": ([], _I),
 }

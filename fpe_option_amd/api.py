@@ -383,6 +383,19 @@ def terminal_states(**columns):
     return [q[i, :nd[i]].copy() for i in range(n)], st
 
 
+def apply_operators(x, **columns):
+    """K x and M x (unscaled) of one vanilla or out option's domain by the
+    sum-factorised Kronecker products on the GPU."""
+    arr = _batch_array(columns)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    Kx, Mx = np.zeros_like(x), np.zeros_like(x)
+    rc = L.lib().fpe_batch_apply_operators(arr, len(x), x.ctypes.data_as(L._DP), Kx.ctypes.data_as(L._DP),
+                                           Mx.ctypes.data_as(L._DP))
+    if rc != 0:
+        raise RuntimeError(f"fpe_batch_apply_operators failed ({rc}): {L.last_error()}")
+    return Kx, Mx
+
+
 def operators(**columns):
     """GPU-assembled M and K (unscaled) of one vanilla or out option as 49-point
     stencils (N x 49 each) and the grid (n_sb, n_vb)."""

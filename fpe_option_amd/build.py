@@ -36,7 +36,7 @@ def build(verbose=False):
         o = os.path.join(OUT_DIR, src + ".o")
         _run([CXX] + CXX_FLAGS + ["-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
-    for src in ("engine.hip", "surrogate.hip"):
+    for src in ("engine.hip", "surrogate.hip", "pod.hip"):
         o = os.path.join(OUT_DIR, src + ".o")
         msg = _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", o])
         if verbose and msg:

@@ -615,6 +615,39 @@ int32_t fpe_batch_operators(const FpeBatchOption* opt, int32_t cap_dof, double* 
     return 0;
 }
 
+int32_t fpe_batch_apply_operators(const FpeBatchOption* opt, int32_t n_dof, const double* x, double* Kx,
+                                  double* Mx) {
+    g_last_error.clear();
+    if (!opt || !x || !Kx || !Mx || n_dof <= 0) {
+        g_last_error = "invalid arguments";
+        return 1;
+    }
+    fpe::OptionSpec o = from_batch(*opt);
+    std::string why;
+    if (fpe::is_in_option(o.option_type) || !fpe::option_is_valid(o, &why)) {
+        g_last_error = "a valid vanilla or out option is required: " + why;
+        return 1;
+    }
+    fpe::SolveSpec sp = fpe::solve_for(o, o.option_type);
+    fpe::Domain d = fpe::build_domain(sp);
+    if (!fpe::grid_fits(d, &g_last_error)) return 5;
+    if (d.n_sb * d.n_vb != n_dof) {
+        g_last_error = "n_dof does not match the grid (" + std::to_string(d.n_sb * d.n_vb) + " dofs)";
+        return 1;
+    }
+    std::vector<double> xv(x, x + n_dof);
+    std::vector<fpe::SolveResult> res;
+    fpe::BatchTimes bt;
+    std::vector<double> dummy;
+    fpe::RunOptions ro;
+    ro.apply_x = &xv;
+    ro.stop_after_ic = true;
+    if (!fpe::run_batch({sp}, {d}, {}, 0, ro, dummy, res, bt, g_last_error)) return 2;
+    std::copy(res[0].Kx.begin(), res[0].Kx.end(), Kx);
+    std::copy(res[0].Mx.begin(), res[0].Mx.end(), Mx);
+    return 0;
+}
+
 int32_t fpe_european_pricing_operator(const FpeBatchOption* opt, int32_t cap_qs, int32_t cap_dof, double* s_phys,
                                       double* marg, int32_t* n_qs, int32_t* n_dof) {
     g_last_error.clear();

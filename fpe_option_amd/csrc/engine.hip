@@ -13,6 +13,7 @@
 #include "assemble.inl"
 #include "engine.hpp"
 #include "ic.inl"
+#include "kron.inl"
 #include "payoff.inl"
 #include "radau.inl"
 
@@ -64,7 +65,7 @@ struct DeviceState {
     hipStream_t stream = nullptr;
     hipEvent_t ev[8];
     unsigned char* scratch = nullptr;  // device address of `scratch` on this device
-    DBuf inst, out, kn, qd, bv, bc, fc, st, vc, eg, hs, wm, ow, ws, wi, req, part, queue, order, pdf;
+    DBuf inst, out, kn, qd, bv, bc, fc, st, vc, eg, hs, wm, ow, ws, wi, req, part, queue, order, pdf, kron;
 };
 
 // Device pointers of one batch, from the grow-only buffers or from the scratch.
@@ -482,6 +483,22 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
             R.q0.resize(N);
             HIP_OK(hipMemcpy(R.q0.data(), D.vc + hi[s].o_vec + V_Q0 * N, sizeof(double) * N,
                              hipMemcpyDeviceToHost));
+        }
+        if (opt.apply_x && s == 0 && !given) {
+            if (opt.apply_x->size() != size_t(N)) {
+                err = "apply_x has the wrong length";
+                return false;
+            }
+            if (!G.kron.ensure(sizeof(double) * 8 * N, err)) return false;
+            double* kb = G.kron.as<double>();
+            HIP_OK(hipMemcpyAsync(kb, opt.apply_x->data(), sizeof(double) * N, hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_kron_apply, dim3(1), dim3(NT), 0, st, dI, 0, D.fc, kb, kb + 3 * N, kb + N, kb + 2 * N);
+            HIP_OK(hipGetLastError());
+            R.Kx.resize(N);
+            R.Mx.resize(N);
+            HIP_OK(hipMemcpyAsync(R.Kx.data(), kb + N, sizeof(double) * N, hipMemcpyDeviceToHost, st));
+            HIP_OK(hipMemcpyAsync(R.Mx.data(), kb + 2 * N, sizeof(double) * N, hipMemcpyDeviceToHost, st));
+            HIP_OK(hipStreamSynchronize(st));
         }
         if (opt.want_stencils && !given) {
             // unscaled M and K as N x 49 rows (entry (dJ + 3) * 7 + dj + 3), from the

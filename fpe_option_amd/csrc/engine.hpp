@@ -20,6 +20,7 @@ struct SolveResult {
     std::vector<std::vector<double>> hist;   // accepted states (stepwise API)
     std::vector<double> pdf;                 // n_qs x n_qv (stepwise API)
     std::vector<double> M49, K49;            // unscaled stencils, N x 49 (operator export)
+    std::vector<double> Kx, Mx;              // sum-factorised K x, M x (RunOptions::apply_x)
     std::vector<double> bas[2];              // per-point basis tables (8 doubles) for s, v
     std::vector<int> bas_c0[2];              // first recombined column per point
 };
@@ -50,6 +51,9 @@ struct RunOptions {
     const std::vector<std::vector<double>>* given_qT = nullptr;
     // no device allocation: all buffers are slices of the fixed scratch array (pricing only)
     bool noalloc = false;
+    // when set (N values for solve 0): K x and M x of solve 0 by the sum-factorised
+    // products, unscaled (SolveResult::Kx, Mx)
+    const std::vector<double>* apply_x = nullptr;
 };
 
 // Largest block-row count the kernels take (the v block width is bounded by MAX_BLOCK = 64).

@@ -1,5 +1,5 @@
-// Surrogate kernels: a multilayer perceptron on MFMA with BF16 operands and
-// FP32 accumulation (forward, backward, Adam, loss), and pricing from POD
+// Surrogate kernels: a multilayer perceptron on MFMA, with BF16 operands or
+// FP32 operands and FP32 accumulation (forward, backward, Adam, loss), and pricing from POD
 // coefficients of the terminal density (coefficients -> s-marginal -> payoff).
 // Nothing here has a counterpart in the reference engine; the plain
 // Fokker-Planck solve is the surrogate's fallback and its accuracy reference.
@@ -62,16 +62,70 @@ __device__ __forceinline__ void wave_gemm(int r0, int c0, int M, int N, int K, F
     }
 }
 
+// FP32 operands: acc[t] = C[r0 .. r0+15][c0 + 16t .. c0 + 16t + 15] for t = 0..3,
+// C = A B with inner dimension K on v_mfma_f32_16x16x4f32 (A(m, k), B(k, n)
+// return floats).
+template <class FA, class FB>
+__device__ __forceinline__ void wave_gemm_f32(int r0, int c0, int M, int N, int K, FA A, FB B, f4 acc[4]) {
+    const int l = threadIdx.x & 63, lr = l & 15, lk = l >> 4;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+    const int m = r0 + lr;
+    for (int k0 = 0; k0 < K; k0 += 4) {
+        const int k = k0 + lk;
+        const float a = (m < M && k < K) ? A(m, k) : 0.0f;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int n = c0 + 16 * t + lr;
+            const float b = (n < N && k < K) ? B(k, n) : 0.0f;
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[t], 0, 0, 0);
+        }
+    }
+}
+
+// Weight element n, k as a BF16 bit pattern (BF16 path) or a float (FP32 path).
+template <bool F32>
+struct WeightT;
+template <>
+struct WeightT<false> {
+    typedef unsigned short type;
+};
+template <>
+struct WeightT<true> {
+    typedef float type;
+};
+
+// C = A B over the BF16 operands of the BF16 path (X and dZ rounded to BF16) or
+// the FP32 operands of the FP32 path.
+template <bool F32, class FA, class FB>
+__device__ __forceinline__ void wave_gemm_p(int r0, int c0, int M, int N, int K, FA A, FB B, f4 acc[4]) {
+    if constexpr (F32)
+        wave_gemm_f32(r0, c0, M, N, K, A, B, acc);
+    else
+        wave_gemm(
+            r0, c0, M, N, K, [&](int m, int k) { return f2bf(A(m, k)); }, [&](int k, int n) { return f2bf(B(k, n)); },
+            acc);
+}
+template <bool F32>
+__device__ __forceinline__ float wval(const typename WeightT<F32>::type* W, long long i) {
+    if constexpr (F32)
+        return W[i];
+    else
+        return __uint_as_float(unsigned(W[i]) << 16);  // exact: BF16 -> FP32 -> BF16 round trips
+}
+
 // Z = X W^T + b, A = act(Z). grid (ceil(M/64), ceil(N/64)).
+template <bool F32>
 __global__ void __launch_bounds__(NT) k_lin_fwd(const float* __restrict__ X, int M, int K,
-                                                const unsigned short* __restrict__ Wb, const float* __restrict__ bias,
-                                                int N, int act, float* __restrict__ Z, float* __restrict__ Aout) {
+                                                const typename WeightT<F32>::type* __restrict__ W,
+                                                const float* __restrict__ bias, int N, int act, float* __restrict__ Z,
+                                                float* __restrict__ Aout) {
     const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int r0 = blockIdx.x * TILE + 16 * w, c0 = blockIdx.y * TILE;
     f4 acc[4];
-    wave_gemm(
-        r0, c0, M, N, K, [&](int m, int k) { return f2bf(X[(long long)m * K + k]); },
-        [&](int k, int n) { return Wb[(long long)n * K + k]; }, acc);
+    wave_gemm_p<F32>(
+        r0, c0, M, N, K, [&](int m, int k) { return X[(long long)m * K + k]; },
+        [&](int k, int n) { return wval<F32>(W, (long long)n * K + k); }, acc);
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         const int n = c0 + 16 * t + (l & 15);
@@ -88,15 +142,16 @@ __global__ void __launch_bounds__(NT) k_lin_fwd(const float* __restrict__ X, int
 }
 
 // dX = dZ W (M x K), times silu'(Zp) when Zp != nullptr. grid (ceil(M/64), ceil(K/64)).
-__global__ void __launch_bounds__(NT) k_lin_bwd_data(const float* __restrict__ dZ, const unsigned short* __restrict__ Wb,
-                                                     int M, int N, int K, const float* __restrict__ Zp,
-                                                     float* __restrict__ dX) {
+template <bool F32>
+__global__ void __launch_bounds__(NT) k_lin_bwd_data(const float* __restrict__ dZ,
+                                                     const typename WeightT<F32>::type* __restrict__ W, int M, int N,
+                                                     int K, const float* __restrict__ Zp, float* __restrict__ dX) {
     const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int r0 = blockIdx.x * TILE + 16 * w, c0 = blockIdx.y * TILE;
     f4 acc[4];
-    wave_gemm(
-        r0, c0, M, K, N, [&](int m, int n) { return f2bf(dZ[(long long)m * N + n]); },
-        [&](int n, int k) { return Wb[(long long)n * K + k]; }, acc);
+    wave_gemm_p<F32>(
+        r0, c0, M, K, N, [&](int m, int n) { return dZ[(long long)m * N + n]; },
+        [&](int n, int k) { return wval<F32>(W, (long long)n * K + k); }, acc);
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         const int k = c0 + 16 * t + (l & 15);
@@ -116,14 +171,15 @@ __global__ void __launch_bounds__(NT) k_lin_bwd_data(const float* __restrict__ d
 }
 
 // dW = dZ^T X (N x K). grid (ceil(N/64), ceil(K/64)).
+template <bool F32>
 __global__ void __launch_bounds__(NT) k_lin_bwd_weight(const float* __restrict__ dZ, const float* __restrict__ X,
                                                        int M, int N, int K, float* __restrict__ dW) {
     const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int r0 = blockIdx.x * TILE + 16 * w, c0 = blockIdx.y * TILE;
     f4 acc[4];
-    wave_gemm(
-        r0, c0, N, K, M, [&](int n, int m) { return f2bf(dZ[(long long)m * N + n]); },
-        [&](int m, int k) { return f2bf(X[(long long)m * K + k]); }, acc);
+    wave_gemm_p<F32>(
+        r0, c0, N, K, M, [&](int n, int m) { return dZ[(long long)m * N + n]; },
+        [&](int m, int k) { return X[(long long)m * K + k]; }, acc);
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         const int k = c0 + 16 * t + (l & 15);
@@ -275,34 +331,57 @@ int32_t bad(const std::string& msg) {
     return 1;
 }
 
+template <bool F32, class WT>
+int32_t mlp_forward(const float* X, int32_t M, int32_t K, const WT* W, const float* b, int32_t N, int32_t act,
+                    float* Z, float* A, void* stream, const char* name) {
+    if (M < 0 || K <= 0 || N <= 0 || !X || !W || !b) return bad(std::string(name) + ": invalid arguments");
+    if (M == 0) return 0;
+    dim3 g((M + TILE - 1) / TILE, (N + TILE - 1) / TILE);
+    hipLaunchKernelGGL(k_lin_fwd<F32>, g, dim3(NT), 0, (hipStream_t)stream, X, M, K, W, b, N, act, Z, A);
+    return check(name);
+}
+
+template <bool F32, class WT>
+int32_t mlp_backward(const float* dZ, const float* X, const WT* W, int32_t M, int32_t N, int32_t K, float* dW,
+                     float* db, float* dX, const float* Zp, void* stream, const char* name) {
+    if (M <= 0 || K <= 0 || N <= 0 || !dZ || !X || !W || !dW || !db)
+        return bad(std::string(name) + ": invalid arguments");
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_lin_bwd_weight<F32>, dim3((N + TILE - 1) / TILE, (K + TILE - 1) / TILE), dim3(NT), 0, st, dZ,
+                       X, M, N, K, dW);
+    hipLaunchKernelGGL(k_colsum, dim3((N + NT - 1) / NT), dim3(NT), 0, st, dZ, M, N, db);
+    if (dX)
+        hipLaunchKernelGGL(k_lin_bwd_data<F32>, dim3((M + TILE - 1) / TILE, (K + TILE - 1) / TILE), dim3(NT), 0, st,
+                           dZ, W, M, N, K, Zp, dX);
+    return check(name);
+}
 }  // namespace fpe_sur
 
 using namespace fpe_sur;
 
 extern "C" {
 
+
 int32_t fpe_mlp_forward(const float* X, int32_t M, int32_t K, const uint16_t* Wb, const float* b, int32_t N,
                         int32_t act, float* Z, float* A, void* stream) {
-    if (M < 0 || K <= 0 || N <= 0 || !X || !Wb || !b) return bad("fpe_mlp_forward: invalid arguments");
-    if (M == 0) return 0;
-    dim3 g((M + TILE - 1) / TILE, (N + TILE - 1) / TILE);
-    hipLaunchKernelGGL(k_lin_fwd, g, dim3(NT), 0, (hipStream_t)stream, X, M, K,
-                       reinterpret_cast<const unsigned short*>(Wb), b, N, act, Z, A);
-    return check("k_lin_fwd");
+    return mlp_forward<false>(X, M, K, reinterpret_cast<const unsigned short*>(Wb), b, N, act, Z, A, stream,
+                              "fpe_mlp_forward");
 }
 
 int32_t fpe_mlp_backward(const float* dZ, const float* X, const uint16_t* Wb, int32_t M, int32_t N, int32_t K,
                          float* dW, float* db, float* dX, const float* Zp, void* stream) {
-    if (M <= 0 || K <= 0 || N <= 0 || !dZ || !X || !Wb || !dW || !db)
-        return bad("fpe_mlp_backward: invalid arguments");
-    hipStream_t st = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_lin_bwd_weight, dim3((N + TILE - 1) / TILE, (K + TILE - 1) / TILE), dim3(NT), 0, st, dZ, X,
-                       M, N, K, dW);
-    hipLaunchKernelGGL(k_colsum, dim3((N + NT - 1) / NT), dim3(NT), 0, st, dZ, M, N, db);
-    if (dX)
-        hipLaunchKernelGGL(k_lin_bwd_data, dim3((M + TILE - 1) / TILE, (K + TILE - 1) / TILE), dim3(NT), 0, st, dZ,
-                           reinterpret_cast<const unsigned short*>(Wb), M, N, K, Zp, dX);
-    return check("fpe_mlp_backward");
+    return mlp_backward<false>(dZ, X, reinterpret_cast<const unsigned short*>(Wb), M, N, K, dW, db, dX, Zp, stream,
+                               "fpe_mlp_backward");
+}
+
+int32_t fpe_mlp_forward_f32(const float* X, int32_t M, int32_t K, const float* W, const float* b, int32_t N,
+                            int32_t act, float* Z, float* A, void* stream) {
+    return mlp_forward<true>(X, M, K, W, b, N, act, Z, A, stream, "fpe_mlp_forward_f32");
+}
+
+int32_t fpe_mlp_backward_f32(const float* dZ, const float* X, const float* W, int32_t M, int32_t N, int32_t K,
+                             float* dW, float* db, float* dX, const float* Zp, void* stream) {
+    return mlp_backward<true>(dZ, X, W, M, N, K, dW, db, dX, Zp, stream, "fpe_mlp_backward_f32");
 }
 
 int32_t fpe_mlp_adam(float* W, uint16_t* Wb, const float* G, float* m, float* v, int64_t n, float lr, float beta1,

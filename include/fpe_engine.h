@@ -266,6 +266,13 @@ int32_t fpe_batch_terminal_states(const FpeBatchOption* opts, int32_t n, double*
 int32_t fpe_batch_operators(const FpeBatchOption* opt, int32_t cap_dof, double* M49, double* K49, int32_t* n_sb,
                             int32_t* n_vb);
 
+/** K x and M x for one vanilla or out option's domain (unscaled operators as in
+ *  fpe_batch_operators) by the sum-factorised Kronecker products from the 1D
+ *  factors (src/sparse/kron_spmv.mojo:13-146): x, Kx, Mx hold n_dof = n_sb n_vb
+ *  values, dof r = I n_vb + i. Returns 0, FPE_TOO_LARGE, or nonzero on error. */
+int32_t fpe_batch_apply_operators(const FpeBatchOption* opt, int32_t n_dof, const double* x, double* Kx,
+                                  double* Mx);
+
 /** Pricing operator of one European domain: the s quadrature points s_phys
  *  (n_qs) and the marginal operator marg (n_qs x n_dof, row-major) with
  *  price = exp(-r T) sum_i payoff(s_phys[i]) (marg q(T))_i, i.e.
@@ -289,6 +296,13 @@ int32_t fpe_mlp_forward(const float* X, int32_t M, int32_t K, const uint16_t* Wb
  *  NULL. BF16 operands, FP32 accumulation. */
 int32_t fpe_mlp_backward(const float* dZ, const float* X, const uint16_t* Wb, int32_t M, int32_t N, int32_t K,
                          float* dW, float* db, float* dX, const float* Zp, void* stream);
+
+/** fpe_mlp_forward / fpe_mlp_backward with FP32 weights W (N x K) and FP32
+ *  operands on v_mfma_f32_16x16x4f32 (no BF16 rounding), FP32 accumulation. */
+int32_t fpe_mlp_forward_f32(const float* X, int32_t M, int32_t K, const float* W, const float* b, int32_t N,
+                            int32_t act, float* Z, float* A, void* stream);
+int32_t fpe_mlp_backward_f32(const float* dZ, const float* X, const float* W, int32_t M, int32_t N, int32_t K,
+                             float* dW, float* db, float* dX, const float* Zp, void* stream);
 
 /** Adam step (bias-corrected, step t >= 1) on FP32 master weights W (n), then
  *  Wb = BF16(W) when Wb is not NULL. */
@@ -323,6 +337,25 @@ int32_t fpe_mlp_to_bf16(const float* W, uint16_t* Wb, int64_t n, void* stream);
 int32_t fpe_surrogate_price(const float* C, int32_t n, int32_t ncoef, const float* cscale, const float* cshift,
                             const float* Pt, const float* w0, const float* sq, int32_t nq, const double* strike,
                             const int32_t* type, const double* r, const double* T, double* prices, void* stream);
+
+/** POD of q(T) snapshots in the pricing inner product (no counterpart in the
+ *  reference: the surrogate's basis). Device pointers: Q (n_snap x n_dof,
+ *  row-major, 1 <= n_snap <= 16384), G the s-marginal operator (n_q x n_dof).
+ *  Outputs: mean (n_dof), Phi (n_dof x k), P (n_q x k, orthonormal columns),
+ *  w0 = G mean (n_q), sv (n_q) the singular values of (Q - mean) G^T,
+ *  descending. Mode signs make each P column's largest-magnitude entry
+ *  positive. n_sweeps (host, may be NULL) receives the Jacobi sweep count.
+ *  Returns 0, 1 (invalid arguments), 2 (HIP error) or 3 (no convergence). */
+int32_t fpe_pod_fit(const double* Q, int32_t n_snap, int32_t n_dof, const double* G, int32_t n_q, int32_t k,
+                    double* mean, double* Phi, double* P, double* w0, double* sv, int32_t* n_sweeps, void* stream);
+
+/** Singular values and right singular vectors of A (n_rows x n_cols,
+ *  column-major with leading dimension lda, device memory, n_rows <= 16384)
+ *  by one-sided Jacobi; A is overwritten. sv (n_cols) descending; V
+ *  (n_cols x n_cols, row-major) holds the matching vectors as columns, signed
+ *  as in fpe_pod_fit. No counterpart in the reference. */
+int32_t fpe_svd_jacobi(double* A, int32_t n_rows, int32_t n_cols, int32_t lda, double* sv, double* V,
+                       int32_t* n_sweeps, void* stream);
 
 /** Human-readable description of the last failure in this thread (empty if none). */
 const char* fpe_last_error(void);

@@ -19,15 +19,19 @@ def dsilu(z):
     return s * (1.0 + z * (1.0 - s))
 
 
-def linear_forward(X, W, b, act):
-    Z = bf16(X) @ bf16(W).T + b
+def _r(x, fp32):
+    return np.asarray(x, np.float64) if fp32 else bf16(x)
+
+
+def linear_forward(X, W, b, act, fp32=False):
+    Z = _r(X, fp32) @ _r(W, fp32).T + b
     return Z, (silu(Z) if act else Z)
 
 
-def linear_backward(dZ, X, W, Zp):
-    dW = bf16(dZ).T @ bf16(X)
+def linear_backward(dZ, X, W, Zp, fp32=False):
+    dW = _r(dZ, fp32).T @ _r(X, fp32)
     db = dZ.sum(axis=0)
-    dX = bf16(dZ) @ bf16(W)
+    dX = _r(dZ, fp32) @ _r(W, fp32)
     if Zp is not None:
         dX = dX * dsilu(Zp)
     return dW, db, dX

@@ -27,3 +27,29 @@ def test_calibration_recovers_parameters(engine):
     # the fit improved on the starting point
     start = cal.model_prices(init[:, None, :], K, T, BASE)[:, 0] - market
     assert np.all(np.sum(out["residuals"] ** 2, axis=1) < np.sum(start ** 2, axis=1))
+
+
+def test_calibration_recovers_parameters_from_oracle_market(engine, oracle):
+    """The market is priced by the FP64 oracle, not by this engine: the GPU
+    calibration has to land on the oracle's parameters, so it is pinned to the
+    oracle through the prices (the GPU and oracle prices agree to ~1e-8)."""
+    truth = np.array([[1.1, 0.06, 0.3, -0.35, 0.05], [2.0, 0.04, 0.5, -0.6, 0.08]])
+    strikes = [85.0, 95.0, 100.0, 105.0, 115.0]
+    expiries = [0.25, 0.75]
+    K = np.tile(np.repeat([strikes], len(expiries), axis=0).reshape(-1), (len(truth), 1))
+    T = np.tile(np.repeat(expiries, len(strikes)), (len(truth), 1))
+    kw = {k: v for k, v in BASE.items()}
+    market = np.zeros_like(K)
+    for b, x in enumerate(truth):
+        for j, t in enumerate(expiries):
+            p, st, rc = oracle.price(strikes, option_type=8, barrier=0.0, T=t, kappa=x[0], theta=x[1], sigma=x[2],
+                                  rho=x[3], V0=x[4], **kw)
+            assert rc == 0
+            market[b, j * len(strikes):(j + 1) * len(strikes)] = p
+    init = truth * np.array([1.02, 0.98, 1.02, 0.98, 1.02])
+    out = cal.calibrate_batch(market, K, T, init, base=BASE, max_iter=30, tol=1e-12)
+    rel = np.abs(out["params"] - truth) / np.abs(truth)
+    print("oracle market: relative parameter errors", rel.max(axis=1), "iterations", out["iterations"],
+          "max residual", np.max(np.abs(out["residuals"])))
+    assert np.max(np.abs(out["residuals"])) < 1e-7
+    assert np.all(rel < 1e-3)

@@ -87,6 +87,23 @@ def test_grid_shapes_against_fixture(engine, mode, lu_variant):
     _check(rows, g, status, info, qT, cases, f"grids_{mode}_lu{lu_variant}")
 
 
+def test_sum_factorised_products_against_fixture(engine):
+    """K x and M x by the sum-factorised Kronecker products (the GMRES operator's
+    form) against the oracle's assembled operators times x, to 1e-13 of
+    max|A| max|x| per entry."""
+    import oracle_lib
+    from fpe_option_amd import api
+    rng = np.random.default_rng(5)
+    for name, c in golden_data.load("stencils").items():
+        ns, nv = c["dims"]
+        x = rng.standard_normal(ns * nv)
+        Kx, Mx = api.apply_operators(x, **c["kw"])
+        for got, S49, lab in ((Kx, c["K"], "K"), (Mx, c["M"], "M")):
+            A = oracle_lib.dense_to_stencil_matrix(S49, ns, nv)
+            err = np.max(np.abs(got - A @ x)) / (np.max(np.abs(A)) * np.max(np.abs(x)))
+            assert err <= 1e-13, f"{name} {lab}: {err:.2e}"
+
+
 def test_operators_against_fixture(engine):
     """Mass and stiffness operators as assembled on the GPU (de Boor basis,
     1D Gram factors, Kronecker sums: src/engines/fpe/galerkin.mojo:16-125)

@@ -27,38 +27,44 @@ def _close(g, r, tol):
     assert err <= tol, f"max err / max |ref| = {err:.3e}"
 
 
+@pytest.mark.parametrize("fp32", [False, True])
 @pytest.mark.parametrize("M,K,N,act", [(37, 7, 50, 1), (128, 256, 256, 1), (70, 256, 64, 0), (1, 3, 5, 1)])
-def test_linear_forward_and_backward(dev, M, K, N, act):
+def test_linear_forward_and_backward(dev, M, K, N, act, fp32):
     S = dev
     rng = np.random.default_rng(M * 1000 + K)
     X = rng.standard_normal((M, K)).astype(np.float32)
     W = (rng.standard_normal((N, K)) / np.sqrt(K)).astype(np.float32)
     b = rng.standard_normal(N).astype(np.float32) * 0.1
-    net = S.DeviceMLP([W], [b])
+    net = S.DeviceMLP([W], [b], fp32)
     lib, st = net.lib, net.stream
+    fwd, bwd = (lib.fpe_mlp_forward_f32, lib.fpe_mlp_backward_f32) if fp32 else (lib.fpe_mlp_forward,
+                                                                                  lib.fpe_mlp_backward)
+    Wl = net.W[0] if fp32 else net.Wb[0]
     Xd, Z, A = _t(S, X), S.DeviceArray((M, N), np.float32), S.DeviceArray((M, N), np.float32)
-    assert lib.fpe_mlp_forward(Xd.c, M, K, net.Wb[0].c, net.b[0].c, N, act, Z.c, A.c, st) == 0
-    Zr, Ar = R.linear_forward(X, W, b.astype(np.float64), act)
+    assert fwd(Xd.c, M, K, Wl.c, net.b[0].c, N, act, Z.c, A.c, st) == 0
+    Zr, Ar = R.linear_forward(X, W, b.astype(np.float64), act, fp32)
     _close(Z.numpy(), Zr, 1e-5)
     _close(A.numpy(), Ar, 1e-5)
     dZ = rng.standard_normal((M, N)).astype(np.float32)
     Zp = rng.standard_normal((M, K)).astype(np.float32)
     dW, db, dX = S.DeviceArray((N, K), np.float32), S.DeviceArray((N,), np.float32), S.DeviceArray((M, K), np.float32)
     dZd, Zpd = _t(S, dZ), _t(S, Zp)
-    assert lib.fpe_mlp_backward(dZd.c, Xd.c, net.Wb[0].c, M, N, K, dW.c, db.c, dX.c, Zpd.c, st) == 0
-    dWr, dbr, dXr = R.linear_backward(dZ, X, W, Zp.astype(np.float64))
+    assert bwd(dZd.c, Xd.c, Wl.c, M, N, K, dW.c, db.c, dX.c, Zpd.c, st) == 0
+    dWr, dbr, dXr = R.linear_backward(dZ, X, W, Zp.astype(np.float64), fp32)
     _close(dW.numpy(), dWr, 1e-5)
     _close(db.numpy(), dbr, 1e-5)
     _close(dX.numpy(), dXr, 1e-5)
 
 
-def test_network_gradients_match_reference(dev):
+@pytest.mark.parametrize("fp32", [False, True])
+def test_network_gradients_match_reference(dev, fp32):
     S = dev
     rng = np.random.default_rng(7)
     widths = (7, 32, 32, 16)
     Ws = [(rng.standard_normal((b, a)) / np.sqrt(a)).astype(np.float32) for a, b in zip(widths[:-1], widths[1:])]
     bs = [(0.1 * rng.standard_normal(b)).astype(np.float32) for b in widths[1:]]
-    net = S.DeviceMLP(Ws, bs)
+    net = S.DeviceMLP(Ws, bs, fp32)
+    tol = 1e-5 if fp32 else 1e-3
     X = rng.uniform(-1, 1, (64, 7)).astype(np.float32)
     T = rng.standard_normal((64, 16)).astype(np.float32)
     Td = _t(S, T)
@@ -68,18 +74,18 @@ def test_network_gradients_match_reference(dev):
     # reference: forward with the same rounding, MSE gradient, backward
     h, cache = X.astype(np.float64), []
     for i, (W, b) in enumerate(zip(Ws, bs)):
-        Z, A = R.linear_forward(h.astype(np.float32), W, b.astype(np.float64), i < len(Ws) - 1)
+        Z, A = R.linear_forward(h.astype(np.float32), W, b.astype(np.float64), i < len(Ws) - 1, fp32)
         cache.append((h, Z))
         h = A
-    _close(out.numpy(), h, 1e-3)
+    _close(out.numpy(), h, tol)
     loss = np.mean((h - T) ** 2)
-    assert abs(loss_gpu - loss) <= 1e-3 * loss
+    assert abs(loss_gpu - loss) <= tol * loss
     g = 2.0 * (h - T) / h.size
     for i in reversed(range(len(Ws))):
         Zp = cache[i - 1][1] if i > 0 else None
-        dWr, dbr, dX = R.linear_backward(g.astype(np.float32), cache[i][0].astype(np.float32), Ws[i], Zp)
-        _close(grads[i][0].numpy(), dWr, 2e-3)
-        _close(grads[i][1].numpy(), dbr, 2e-3)
+        dWr, dbr, dX = R.linear_backward(g.astype(np.float32), cache[i][0].astype(np.float32), Ws[i], Zp, fp32)
+        _close(grads[i][0].numpy(), dWr, 2 * tol)
+        _close(grads[i][1].numpy(), dbr, 2 * tol)
         g = dX
 
 

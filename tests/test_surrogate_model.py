@@ -1,7 +1,9 @@
 """The trained surrogate (fpe_option_amd/surrogate_80x40.npz) against its accuracy reference,
 the plain FP64 solve on the same 80x40 grid, on fresh seeded parameter sets
-that were not in the training data; and what the network adds over the same
-POD pricing without it (every option priced at the mean coefficients)."""
+that were not in the training data: the served (certified) prices meet the
+1e-4 bar, the stated error estimate covers the error, and the network gains
+over the same POD pricing without it (every option priced at the mean
+coordinates)."""
 import os
 import time
 
@@ -13,6 +15,7 @@ pytestmark = pytest.mark.gpu
 from fpe_option_amd import surrogate as S
 
 needs_model = pytest.mark.skipif(not os.path.exists(S.MODEL_PATH), reason="surrogate model not built")
+TOL = 1e-4
 
 
 @pytest.fixture(scope="module")
@@ -22,7 +25,7 @@ def sur(engine):
 
 @pytest.fixture(scope="module")
 def fresh():
-    n = 96
+    n = 512
     P = S.sample_params(n, 777)  # training used seeds 20261204 and 20261205
     rng = np.random.default_rng(777)
     K = rng.uniform(45.0, 75.0, n)
@@ -30,38 +33,61 @@ def fresh():
     return P, K, ty
 
 
-@needs_model
-def test_surrogate_against_plain_solve(sur, fresh, engine):
+@pytest.fixture(scope="module")
+def reference(fresh, engine):
     P, K, ty = fresh
-    sp, ok = sur.price(P, K, ty)
-    assert ok.all()
     ref, st = S.solver_prices(P, K, ty)
     assert np.all(st == 0)
-    d = np.abs(sp - ref)
-    # the no-network baseline: the same POD pricing at the mean coefficients
+    return ref
+
+
+@needs_model
+def test_certified_prices_meet_the_bar(sur, fresh, reference):
+    P, K, ty = fresh
+    sp, est, ok = sur.price(P, K, ty)
+    assert ok.all()
+    d = np.abs(sp - reference)
+    cert = est <= TOL
+    print(f"surrogate |dprice| max {d.max():.3e} mean {d.mean():.3e}; certified at {TOL}: {cert.sum()}/{len(d)} "
+          f"(max error there {d[cert].max() if cert.any() else 0.0:.2e}); estimate covers the error for "
+          f"{np.mean(d <= est):.1%}")
+    assert np.all(d[cert] <= TOL)
+    assert np.mean(d <= est) >= 0.98
+    cp, cest, cmask = sur.price_certified(P, K, ty, TOL)
+    np.testing.assert_array_equal(cmask, cert)
+    assert np.all(np.isnan(cp[~cmask])) and np.all(np.isfinite(cp[cmask]))
+
+
+@needs_model
+def test_network_gain_over_mean_coordinates(sur, fresh, reference):
+    P, K, ty = fresh
+    sp, _, _ = sur.price(P, K, ty)
+    d = np.abs(sp - reference)
     C0 = S.DeviceArray.from_numpy(np.zeros((len(P), sur.k), np.float32))
     out = S.DeviceArray((len(P),), np.float64)
     args = [S.DeviceArray.from_numpy(a) for a in (K.astype(np.float64), ty, P[:, 5].copy(), P[:, 6].copy())]
     assert S.L.lib().fpe_surrogate_price(C0.c, len(P), sur.k, sur.cscale.c, sur.cshift.c, sur.Pt.c, sur.w0.c,
                                          sur.sq.c, sur.nq, args[0].c, args[1].c, args[2].c, args[3].c, out.c,
                                          None) == 0
-    d0 = np.abs(out.numpy() - ref)
-    print(f"surrogate |dprice| max {d.max():.3e} mean {d.mean():.3e}; without the network max {d0.max():.3e} "
-          f"mean {d0.mean():.3e}")
-    assert np.mean(d) < 0.05 * np.mean(d0)
-    assert np.max(d) < 0.5
+    d0 = np.abs(out.numpy() - reference)
+    print(f"network: mean |dprice| {d.mean():.3e}; mean coordinates: {d0.mean():.3e}")
+    assert np.mean(d) < 0.01 * np.mean(d0)
 
 
 @needs_model
-def test_out_of_box_rows_fall_back_to_the_solver(sur, fresh, engine):
+def test_fallback_to_the_solver(sur, fresh, engine):
     P, K, ty = fresh
     P = P[:6].copy()
     P[1, 2] = 0.95   # sigma above the box
     P[4, 6] = 2.5    # T above the box
-    prices, ok, n_fb = sur.price_with_fallback(P, K[:6], ty[:6])
+    prices, est, ok, n_fb = sur.price_with_fallback(P, K[:6], ty[:6])
     assert n_fb == 2 and not ok[1] and not ok[4]
+    assert est[1] == 0.0 and est[4] == 0.0
     ref, st = S.solver_prices(P[[1, 4]], K[[1, 4]], ty[[1, 4]])
     np.testing.assert_array_equal(prices[[1, 4]], ref)
+    # with a tolerance, uncertified in-box rows go to the solver as well
+    p2, est2, ok2, n2 = sur.price_with_fallback(P, K[:6], ty[:6], tol=0.0)
+    assert n2 == 6 and not ok2.any()
 
 
 @needs_model
