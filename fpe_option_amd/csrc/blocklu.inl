@@ -253,8 +253,9 @@ __device__ __forceinline__ void apply_panel(LuLds& L, int m, int k0, int nb) {
 //  2. every wave applies the steps to its own 16 columns (apply_panel);
 //  3. the panel wave stores the panel columns last.
 // A wave only reads and writes its own columns in 2-3, so two barriers per
-// panel order everything. Sets singular on a zero or non-finite pivot. Ends
-// with a barrier.
+// panel order everything. (One barrier with the panel data double-buffered
+// measured the same: the panel wave's own steps and apply are the chain.)
+// Sets singular on a zero or non-finite pivot. Ends with a barrier.
 template <bool CPLX, class T>
 __device__ __noinline__ void gj_inverse(LuLds& L, int m) {
     m = uniform(m);
@@ -624,245 +625,6 @@ __device__ void schur_pass_gen(GAS T* __restrict__ Pr, GAS T* __restrict__ Pi, l
     }
 }
 
-// L_JI = A_JI Ainv_I for J = I+1..I+nU, in place. A rows come straight from the
-// panels (a wave takes 16-row strips; their fragments stay in registers), the
-// inverse from LDS (B[0], B[1]). TK k-steps and ceil(TK / 4) column tiles are
-// compile-time (tk_class): the MFMA chains run without branches, LDS operand
-// reads are unpredicated, and global loads read clamped addresses (rows past
-// the stack read its last row, columns past m the block's last column) and
-// select zero, so every load of a strip issues at once.
-template <bool CPLX, class T, int TK>
-__device__ void lower_multiply(GAS T* __restrict__ Pr, GAS T* __restrict__ Pi, long long ps, int ld, int I, int m, int nU,
-                               const LuLds& L) {
-    typedef typename Mf<T>::acc A4;
-    constexpr int TN = (TK + 3) / 4;
-    constexpr int TG = CPLX ? 2 : 4;  // column tiles per accumulator group (real)
-    const T* B0 = lds_block<T>(L, 0);
-    const T* B1 = lds_block<T>(L, 1);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int l16 = lane & 15, l4 = lane >> 4;
-    const int M = nU * m, tm = (M + 15) >> 4;
-    for (int t = wave; t < tm; t += RNW) {
-        const int r0 = t << 4;
-        const int ra = min(r0 + l16, M - 1);  // rows past the stack: its last row (not stored)
-        int dJa;
-        const long long oa = stacked_row(ps, ld, I, m, ra, dJa) + (3 - dJa) * m;
-        T ar[TK], ai[TK];
-#pragma unroll
-        for (int s = 0; s < TK; ++s) {
-            const int k = 4 * s + l4;
-            const int kc = min(k, m - 1);
-            const T vr = Pr[oa + kc];
-            const T vi = CPLX ? Pi[oa + kc] : T(0);
-            ar[s] = k < m ? vr : T(0);
-            ai[s] = (CPLX && k < m) ? vi : T(0);
-        }
-        long long oc[4];
-        bool vc[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int r = r0 + Mf<T>::row(lane, j);
-            vc[j] = r < M;
-            int dJ;
-            oc[j] = vc[j] ? stacked_row(ps, ld, I, m, r, dJ) + (3 - dJ) * m : 0;
-        }
-        if constexpr (CPLX) {
-            // complex products with three real ones per k-step (re = p1 - p2,
-            // im = p3 - p1 - p2): three independent chains per column tile
-#pragma unroll
-            for (int u = 0; u < TN; ++u) {
-                const int c = (u << 4) + l16;
-                A4 p1 = A4{0, 0, 0, 0}, p2 = A4{0, 0, 0, 0}, p3 = A4{0, 0, 0, 0};
-                // operand fragments of k-step s + 1 load while step s's MFMAs run
-                T br = B0[l4 * LS + c], bi = B1[l4 * LS + c];
-#pragma unroll
-                for (int s = 0; s < TK; ++s) {
-                    const int kn = 4 * (s + 1) + l4;
-                    T nbr = T(0), nbi = T(0);
-                    if (s + 1 < TK) nbr = B0[kn * LS + c], nbi = B1[kn * LS + c];
-                    p1 = Mf<T>::mfma(ar[s], br, p1);
-                    p2 = Mf<T>::mfma(ai[s], bi, p2);
-                    p3 = Mf<T>::mfma(ar[s] + ai[s], br + bi, p3);
-                    __builtin_amdgcn_sched_barrier(0);
-                    br = nbr, bi = nbi;
-                }
-                if (c < m) {
-#pragma unroll
-                    for (int j = 0; j < 4; ++j)
-                        if (vc[j]) {
-                            Pr[oc[j] + c] = p1[j] - p2[j];
-                            Pi[oc[j] + c] = p3[j] - p1[j] - p2[j];
-                        }
-                }
-            }
-        } else {
-            // TG column tiles of the strip at a time: independent accumulator chains
-#pragma unroll
-            for (int ug = 0; ug < TN; ug += TG) {
-                A4 acc[TG];
-#pragma unroll
-                for (int u = 0; u < TG; ++u) acc[u] = A4{0, 0, 0, 0};
-                T b[TG], nb[TG];
-#pragma unroll
-                for (int u = 0; u < TG; ++u) b[u] = ug + u < TN ? B0[l4 * LS + ((ug + u) << 4) + l16] : T(0);
-#pragma unroll
-                for (int s = 0; s < TK; ++s) {
-                    const int kn = 4 * (s + 1) + l4;
-#pragma unroll
-                    for (int u = 0; u < TG; ++u)
-                        nb[u] = (s + 1 < TK && ug + u < TN) ? B0[kn * LS + ((ug + u) << 4) + l16] : T(0);
-#pragma unroll
-                    for (int u = 0; u < TG; ++u)
-                        if (ug + u < TN) acc[u] = Mf<T>::mfma(ar[s], b[u], acc[u]);
-                    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                    for (int u = 0; u < TG; ++u) b[u] = nb[u];
-                }
-#pragma unroll
-                for (int u = 0; u < TG; ++u) {
-                    const int c = ((ug + u) << 4) + l16;
-                    if (ug + u < TN && c < m) {
-#pragma unroll
-                        for (int j = 0; j < 4; ++j)
-                            if (vc[j]) Pr[oc[j] + c] = acc[u][j];
-                    }
-                }
-            }
-        }
-    }
-}
-
-// A_JK -= L_JI U_IK for J = I+1..I+nU and K = I+1+kb0 .. I+kb0+nkb, with U_IK
-// staged in LDS (B[0] re, B[1] im in complex arithmetic, one block per pass;
-// B[kb] in real). L strips come from the panels into registers once per strip
-// (real: the next strip of the wave is fetched while the current one's chains
-// run); the column tiles of a K block run in groups (C tiles loaded up front,
-// one independent accumulator chain per tile). Compiled for TK k-steps as
-// lower_multiply: branch-free MFMA chains, unpredicated LDS reads, clamped and
-// selected global loads, predicated stores.
-template <bool CPLX, class T, int TK>
-__device__ void schur_pass(GAS T* __restrict__ Pr, GAS T* __restrict__ Pi, long long ps, int ld, int I, int m, int nU,
-                           int kb0, int nkb, const LuLds& L) {
-    typedef typename Mf<T>::acc A4;
-    constexpr int TN = (TK + 3) / 4;
-    constexpr int TG = CPLX ? 2 : 4;  // column tiles per accumulator group
-    constexpr bool PF = false;        // next-strip prefetch: off (its registers made the inlined
-                                      // Gauss-Jordan chain spill)
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int l16 = lane & 15, l4 = lane >> 4;
-    const int M = nU * m, tm = (M + 15) >> 4;
-    auto fetch = [&](int t, T* fr, T* fi) {
-        const int ra = min((t << 4) + l16, M - 1);
-        int dJa;
-        const long long oa = stacked_row(ps, ld, I, m, ra, dJa) + (3 - dJa) * m;
-#pragma unroll
-        for (int s = 0; s < TK; ++s) {
-            const int k = 4 * s + l4;
-            const int kc = min(k, m - 1);
-            const T vr = Pr[oa + kc];
-            const T vi = CPLX ? Pi[oa + kc] : T(0);
-            fr[s] = k < m ? vr : T(0);
-            fi[s] = (CPLX && k < m) ? vi : T(0);
-        }
-    };
-    T ar[TK], ai[TK], nr[TK], ni[TK];
-#pragma unroll
-    for (int s = 0; s < TK; ++s) ar[s] = ai[s] = nr[s] = ni[s] = T(0);
-    if (wave < tm) fetch(wave, ar, ai);
-    for (int t = wave; t < tm; t += RNW) {
-        const int r0 = t << 4;
-        if (PF && t + RNW < tm) fetch(t + RNW, nr, ni);
-        long long oc[4];
-        int dj[4], fi[4];
-        bool vc[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int r = r0 + Mf<T>::row(lane, j);
-            vc[j] = r < M;
-            dj[j] = 1;
-            oc[j] = stacked_row(ps, ld, I, m, min(r, M - 1), dj[j]);  // rows past the stack: loads only
-            fi[j] = min(r, M - 1) - (dj[j] - 1) * m;  // row within block row I + dj
-        }
-        for (int kb = 0; kb < nkb; ++kb) {
-            const T* Br = lds_block<T>(L, CPLX ? 2 * kb : kb);
-            const T* Bi = lds_block<T>(L, CPLX ? 2 * kb + 1 : kb);
-            const int kk = kb0 + kb;  // K = I + 1 + kk; column block d = kk + 4 - dJ in panel I + dJ
-            // A block's first update (step I = max(J, K) - 3, or 0) finds it as
-            // assembled: zero except within 3 of its diagonal, and only those
-            // entries are written beforehand (the first pass writes the whole
-            // block). Known zeros are selected, not taken from memory.
-            int band[4];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) band[j] = (I == 0 || dj[j] == 3 || kk == 2) ? fi[j] : -(1 << 20);
-#pragma unroll
-            for (int ug = 0; ug < TN; ug += TG) {
-                T cr[TG][4], ci[TG][4];
-                A4 accr[TG], acci[TG];
-#pragma unroll
-                for (int u = 0; u < TG; ++u) {
-                    const int c = ((ug + u) << 4) + l16;
-                    const int cc = min(c, m - 1);
-                    accr[u] = A4{0, 0, 0, 0}, acci[u] = A4{0, 0, 0, 0};
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        const bool ok = ug + u < TN && vc[j] && c < m &&
-                                        (band[j] < 0 || (c - band[j] <= 3 && band[j] - c <= 3));
-                        const long long o = oc[j] + (long long)(kk + 4 - dj[j]) * m + cc;
-                        // known zeros (outside the band on a first update) are not loaded
-                        cr[u][j] = ok ? Pr[o] : T(0);
-                        ci[u][j] = (CPLX && ok) ? Pi[o] : T(0);
-                    }
-                }
-                // operand fragments of k-step s + 1 load while step s's MFMAs run
-                T br[TG], bi[TG], nbr[TG], nbi[TG];
-#pragma unroll
-                for (int u = 0; u < TG; ++u) {
-                    const int c = ((ug + u) << 4) + l16;
-                    br[u] = ug + u < TN ? Br[l4 * LS + c] : T(0);
-                    bi[u] = (CPLX && ug + u < TN) ? Bi[l4 * LS + c] : T(0);
-                }
-                __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                for (int s = 0; s < TK; ++s) {
-                    const int kn = 4 * (s + 1) + l4;
-#pragma unroll
-                    for (int u = 0; u < TG; ++u) {
-                        const int c = ((ug + u) << 4) + l16;
-                        const bool ld = s + 1 < TK && ug + u < TN;
-                        nbr[u] = ld ? Br[kn * LS + c] : T(0);
-                        nbi[u] = (CPLX && ld) ? Bi[kn * LS + c] : T(0);
-                    }
-#pragma unroll
-                    for (int u = 0; u < TG; ++u)
-                        if (ug + u < TN) mfma_c(accr[u], acci[u], ar[s], ai[s], br[u], bi[u], CPLX);
-                    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                    for (int u = 0; u < TG; ++u) br[u] = nbr[u], bi[u] = nbi[u];
-                }
-#pragma unroll
-                for (int u = 0; u < TG; ++u) {
-                    const int c = ((ug + u) << 4) + l16;
-                    if (ug + u < TN && c < m) {
-#pragma unroll
-                        for (int j = 0; j < 4; ++j)
-                            if (vc[j]) {
-                                const long long o = oc[j] + (long long)(kk + 4 - dj[j]) * m + c;
-                                Pr[o] = cr[u][j] - accr[u][j];
-                                if (CPLX) Pi[o] = ci[u][j] - acci[u][j];
-                            }
-                    }
-                }
-            }
-        }
-        if (PF) {
-#pragma unroll
-            for (int s2 = 0; s2 < TK; ++s2) ar[s2] = nr[s2], ai[s2] = ni[s2];
-        } else if (t + RNW < tm) {
-            fetch(t + RNW, ar, ai);
-        }
-    }
-}
-
 // Write E = a M_s + h K_s (a = ar + i ai) into the panels from the 49-point
 // stencils, for block_lu: a coalesced zero fill of the blocks it reads before
 // writing them, then the (at most) 49 stencil entries of every row (stencil
@@ -1023,11 +785,253 @@ __device__ __noinline__ void pivot_block_in(LuLds& L, const T* sr, const T* si, 
     stage_in<CPLX, T, 1>(lds_block<T>(L, 0), lds_block<T>(L, 1), uniform(gas(sr)), CPLX ? uniform(gas(si)) : nullptr,
                          ld, m);
 }
+// U block (I, K) of a Schur pass into staging blocks k (re) and k + 1 (im) as a
+// call of its own (same reason as the pivot block's copies).
+template <bool CPLX, class T>
+__device__ __noinline__ void u_block_in(LuLds& L, int k, const T* sr, const T* si, int ld, int m) {
+    m = uniform(m), ld = uniform(ld), k = uniform(k);
+    stage_in<CPLX, T, 1>(lds_block<T>(L, k), CPLX ? lds_block<T>(L, k + 1) : nullptr, uniform(gas(sr)),
+                         CPLX ? uniform(gas(si)) : nullptr, ld, m);
+}
 template <bool CPLX, class T>
 __device__ __noinline__ void pivot_block_out(LuLds& L, T* sr, T* si, int ld, int m) {
     m = uniform(m), ld = uniform(ld);
     stage_out<CPLX, T>(lds_block<T>(L, 0), lds_block<T>(L, 1), uniform(gas(sr)), CPLX ? uniform(gas(si)) : nullptr,
                        ld, m);
+}
+
+// LDS pointer whose value the compiler cannot relate to other addresses: every
+// operand read through it becomes one ds_read with an immediate offset from
+// this one register, instead of a precomputed address per k-step (which the
+// register allocator then spilled to scratch inside the MFMA loops).
+#define LAS __attribute__((address_space(3)))
+template <class T>
+__device__ __forceinline__ LAS const T* lds_base(const T* p) {
+    LAS const T* q = (LAS const T*)p;
+    asm volatile("" : "+v"(q));
+    return q;
+}
+
+// L_JI = A_JI Ainv_I for J = I+1..I+nU, in place, compiled for TK k-steps
+// (tk_class(m)) and ceil(TK / 4) column tiles: branch-free MFMA chains. A rows
+// come straight from the panels (a wave takes 16-row strips; clamped loads,
+// zeros selected past m), the inverse from LDS (B[0], B[1]) through one opaque
+// base per column-tile group with the k-step operand reads at immediate
+// offsets, loaded one k-step ahead. Complex products with three real ones per
+// k-step (re = p1 - p2, im = p3 - p1 - p2): three independent chains per tile.
+template <bool CPLX, class T, int TK>
+__device__ __forceinline__ void lower_multiply(GAS T* __restrict__ Pr, GAS T* __restrict__ Pi, long long ps, int ld,
+                                                int I, int m, int nU, const LuLds& L) {
+    typedef typename Mf<T>::acc A4;
+    constexpr int TN = (TK + 3) / 4;
+    constexpr int TG = CPLX ? 1 : 4;
+    const T* B0 = lds_block<T>(L, 0);
+    const T* B1 = lds_block<T>(L, 1);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int l16 = lane & 15, l4 = lane >> 4;
+    const int M = nU * m, tm = (M + 15) >> 4;
+    for (int t = wave; t < tm; t += RNW) {
+        const int r0 = t << 4;
+        const int ra = min(r0 + l16, M - 1);
+        int dJa;
+        const long long oa = stacked_row(ps, ld, I, m, ra, dJa) + (3 - dJa) * m;
+        T ar[TK], ai[TK];
+#pragma unroll
+        for (int s = 0; s < TK; ++s) {
+            const int k = 4 * s + l4;
+            const int kc = min(k, m - 1);
+            const T vr = Pr[oa + kc];
+            const T vi = CPLX ? Pi[oa + kc] : T(0);
+            ar[s] = k < m ? vr : T(0);
+            ai[s] = (CPLX && k < m) ? vi : T(0);
+        }
+        long long oc[4];
+        bool vc[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int r = r0 + Mf<T>::row(lane, j);
+            vc[j] = r < M;
+            int dJ;
+            oc[j] = vc[j] ? stacked_row(ps, ld, I, m, r, dJ) + (3 - dJ) * m : 0;
+        }
+#pragma unroll
+        for (int ug = 0; ug < TN; ug += TG) {
+            LAS const T* b0 = lds_base(B0 + l4 * LS + (ug << 4) + l16);
+            LAS const T* b1 = lds_base(B1 + l4 * LS + (ug << 4) + l16);
+            A4 p1[TG], p2[TG], p3[TG];
+#pragma unroll
+            for (int u = 0; u < TG; ++u) p1[u] = p2[u] = p3[u] = A4{0, 0, 0, 0};
+            T br[TG], bi[TG];
+#pragma unroll
+            for (int u = 0; u < TG; ++u) {
+                br[u] = ug + u < TN ? b0[16 * u] : T(0);
+                bi[u] = (CPLX && ug + u < TN) ? b1[16 * u] : T(0);
+            }
+#pragma unroll
+            for (int s = 0; s < TK; ++s) {
+                T nbr[TG], nbi[TG];
+#pragma unroll
+                for (int u = 0; u < TG; ++u) {
+                    const bool ldn = s + 1 < TK && ug + u < TN;
+                    nbr[u] = ldn ? b0[4 * (s + 1) * LS + 16 * u] : T(0);
+                    nbi[u] = (CPLX && ldn) ? b1[4 * (s + 1) * LS + 16 * u] : T(0);
+                }
+#pragma unroll
+                for (int u = 0; u < TG; ++u) {
+                    if (ug + u < TN) {
+                        p1[u] = Mf<T>::mfma(ar[s], br[u], p1[u]);
+                        if (CPLX) {
+                            p2[u] = Mf<T>::mfma(ai[s], bi[u], p2[u]);
+                            p3[u] = Mf<T>::mfma(ar[s] + ai[s], br[u] + bi[u], p3[u]);
+                        }
+                    }
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int u = 0; u < TG; ++u) br[u] = nbr[u], bi[u] = nbi[u];
+            }
+#pragma unroll
+            for (int u = 0; u < TG; ++u) {
+                const int c = ((ug + u) << 4) + l16;
+                if (ug + u < TN && c < m) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        if (vc[j]) {
+                            if (CPLX) {
+                                Pr[oc[j] + c] = p1[u][j] - p2[u][j];
+                                Pi[oc[j] + c] = p3[u][j] - p1[u][j] - p2[u][j];
+                            } else {
+                                Pr[oc[j] + c] = p1[u][j];
+                            }
+                        }
+                }
+            }
+        }
+    }
+}
+
+// A_JK -= L_JI U_IK for J = I+1..I+nU and K = I+1+kb0 .. I+kb0+nkb, with U_IK
+// staged in LDS (B[0] re, B[1] im in complex arithmetic, one block per pass;
+// B[kb] in real), compiled for TK k-steps like lower_multiply. An L strip is
+// loaded into registers once per strip and pass; C tiles are loaded before
+// their MFMA chains, which start from zero, and written once. Complex tiles one
+// at a time with three real product chains, real tiles four at a time (four
+// independent chains).
+template <bool CPLX, class T, int TK>
+__device__ __forceinline__ void schur_pass(GAS T* __restrict__ Pr, GAS T* __restrict__ Pi, long long ps, int ld,
+                                            int I, int m, int nU, int kb0, int nkb, const LuLds& L) {
+    typedef typename Mf<T>::acc A4;
+    constexpr int TN = (TK + 3) / 4;
+    constexpr int TG = CPLX ? 1 : 4;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int l16 = lane & 15, l4 = lane >> 4;
+    const int M = nU * m, tm = (M + 15) >> 4;
+    for (int t = wave; t < tm; t += RNW) {
+        const int r0 = t << 4;
+        T ar[TK], ai[TK];
+        {
+            const int ra = min(r0 + l16, M - 1);
+            int dJa;
+            const long long oa = stacked_row(ps, ld, I, m, ra, dJa) + (3 - dJa) * m;
+#pragma unroll
+            for (int s = 0; s < TK; ++s) {
+                const int k = 4 * s + l4;
+                const int kc = min(k, m - 1);
+                const T vr = Pr[oa + kc];
+                const T vi = CPLX ? Pi[oa + kc] : T(0);
+                ar[s] = k < m ? vr : T(0);
+                ai[s] = (CPLX && k < m) ? vi : T(0);
+            }
+        }
+        long long oc[4];
+        int dj[4], fi[4];
+        bool vc[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int r = r0 + Mf<T>::row(lane, j);
+            vc[j] = r < M;
+            dj[j] = 1;
+            oc[j] = stacked_row(ps, ld, I, m, min(r, M - 1), dj[j]);
+            fi[j] = min(r, M - 1) - (dj[j] - 1) * m;
+        }
+        for (int kb = 0; kb < nkb; ++kb) {
+            const T* Br = lds_block<T>(L, CPLX ? 2 * kb : kb);
+            const T* Bi = lds_block<T>(L, CPLX ? 2 * kb + 1 : kb);
+            const int kk = kb0 + kb;
+            // first update of a block: only the entries within 3 of its diagonal are loaded
+            int band[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) band[j] = (I == 0 || dj[j] == 3 || kk == 2) ? fi[j] : -(1 << 20);
+#pragma unroll
+            for (int ug = 0; ug < TN; ug += TG) {
+                T cr[TG][4], ci[TG][4];
+#pragma unroll
+                for (int u = 0; u < TG; ++u) {
+                    const int c = ((ug + u) << 4) + l16;
+                    const int cc = min(c, m - 1);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const bool ok = ug + u < TN && vc[j] && c < m &&
+                                        (band[j] < 0 || (c - band[j] <= 3 && band[j] - c <= 3));
+                        const long long o = oc[j] + (long long)(kk + 4 - dj[j]) * m + cc;
+                        cr[u][j] = ok ? Pr[o] : T(0);
+                        ci[u][j] = (CPLX && ok) ? Pi[o] : T(0);
+                    }
+                }
+                LAS const T* b0 = lds_base(Br + l4 * LS + (ug << 4) + l16);
+                LAS const T* b1 = lds_base(Bi + l4 * LS + (ug << 4) + l16);
+                A4 p1[TG], p2[TG], p3[TG];
+#pragma unroll
+                for (int u = 0; u < TG; ++u) p1[u] = p2[u] = p3[u] = A4{0, 0, 0, 0};
+                T br[TG], bi[TG];
+#pragma unroll
+                for (int u = 0; u < TG; ++u) {
+                    br[u] = ug + u < TN ? b0[16 * u] : T(0);
+                    bi[u] = (CPLX && ug + u < TN) ? b1[16 * u] : T(0);
+                }
+#pragma unroll
+                for (int s = 0; s < TK; ++s) {
+                    T nbr[TG], nbi[TG];
+#pragma unroll
+                    for (int u = 0; u < TG; ++u) {
+                        const bool ldn = s + 1 < TK && ug + u < TN;
+                        nbr[u] = ldn ? b0[4 * (s + 1) * LS + 16 * u] : T(0);
+                        nbi[u] = (CPLX && ldn) ? b1[4 * (s + 1) * LS + 16 * u] : T(0);
+                    }
+#pragma unroll
+                    for (int u = 0; u < TG; ++u) {
+                        if (ug + u < TN) {
+                            p1[u] = Mf<T>::mfma(ar[s], br[u], p1[u]);
+                            if (CPLX) {
+                                p2[u] = Mf<T>::mfma(ai[s], bi[u], p2[u]);
+                                p3[u] = Mf<T>::mfma(ar[s] + ai[s], br[u] + bi[u], p3[u]);
+                            }
+                        }
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int u = 0; u < TG; ++u) br[u] = nbr[u], bi[u] = nbi[u];
+                }
+#pragma unroll
+                for (int u = 0; u < TG; ++u) {
+                    const int c = ((ug + u) << 4) + l16;
+                    if (ug + u < TN && c < m) {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            if (vc[j]) {
+                                const long long o = oc[j] + (long long)(kk + 4 - dj[j]) * m + c;
+                                if (CPLX) {
+                                    Pr[o] = cr[u][j] - (p1[u][j] - p2[u][j]);
+                                    Pi[o] = ci[u][j] - (p3[u][j] - p1[u][j] - p2[u][j]);
+                                } else {
+                                    Pr[o] = cr[u][j] - p1[u][j];
+                                }
+                            }
+                    }
+                }
+            }
+        }
+    }
 }
 
 // Factor the panels in place (generic loops). Returns nonzero if a diagonal block was singular.
@@ -1106,8 +1110,6 @@ __device__ __noinline__ int block_lu_tk(T* __restrict__ Pr_, T* __restrict__ Pi_
     };
     // U blocks per Schur pass: 2 fit the staging buffers in real arithmetic, 1 in complex
     constexpr int KPASS = CPLX ? 1 : 2;
-    T* B0 = lds_block<T>(L, 0);
-    T* B1 = lds_block<T>(L, 1);
     for (int I = 0; I < ns; ++I) {
         GAS T* PIr = Pr + I * ps;
         GAS T* PIi = Pi + I * ps;
@@ -1125,10 +1127,9 @@ __device__ __noinline__ int block_lu_tk(T* __restrict__ Pr_, T* __restrict__ Pi_
         for (int kb0 = 0; kb0 < nU; kb0 += KPASS) {
             const int nkb = (nU - kb0) < KPASS ? (nU - kb0) : KPASS;
             for (int kb = 0; kb < nkb; ++kb) {
-                T* Br = lds_block<T>(L, CPLX ? 2 * kb : kb);
-                T* Bi = CPLX ? lds_block<T>(L, 2 * kb + 1) : nullptr;
                 const int col = (4 + kb0 + kb) * m;
-                stage_in<CPLX, T, 1>(Br, Bi, PIr + col, CPLX ? PIi + col : nullptr, ld, m);
+                u_block_in<CPLX, T>(L, CPLX ? 2 * kb : kb, (const T*)(PIr + col), CPLX ? (const T*)(PIi + col) : nullptr,
+                                    ld, m);
             }
             __syncthreads();
             lap(1);
