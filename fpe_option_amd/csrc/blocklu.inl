@@ -910,6 +910,17 @@ __device__ __forceinline__ void lower_multiply(GAS T* __restrict__ Pr, GAS T* __
     }
 }
 
+// The U block of the last Schur pass, U_{I,I+3}, still holds only its
+// assembled band (|k - j| <= 3): block (I, I+3) is updated by the steps p with
+// I <= p <= I - 1, i.e. by none. Column tile u (columns 16 u .. 16 u + 15)
+// then needs only the k-steps that meet rows 16 u - 3 .. 16 u + 18, 4 u - 1 to
+// 4 u + 4: six of the TK k-steps. The skipped k-steps' products are exact zeros,
+// so the result is the same as the full loop's.
+template <bool BAND>
+__device__ __forceinline__ constexpr bool kstep_in_band(int u, int s) {
+    return !BAND || (s >= 4 * u - 1 && s <= 4 * u + 4);
+}
+
 // A_JK -= L_JI U_IK for J = I+1..I+nU and K = I+1+kb0 .. I+kb0+nkb, with U_IK
 // staged in LDS (B[0] re, B[1] im in complex arithmetic, one block per pass;
 // B[kb] in real), compiled for TK k-steps like lower_multiply. An L strip is
@@ -917,7 +928,7 @@ __device__ __forceinline__ void lower_multiply(GAS T* __restrict__ Pr, GAS T* __
 // their MFMA chains, which start from zero, and written once. Complex tiles one
 // at a time with three real product chains, real tiles four at a time (four
 // independent chains).
-template <bool CPLX, class T, int TK>
+template <bool CPLX, class T, int TK, bool BAND = false>
 __device__ __forceinline__ void schur_pass(GAS T* __restrict__ Pr, GAS T* __restrict__ Pi, long long ps, int ld,
                                             int I, int m, int nU, int kb0, int nkb, const LuLds& L) {
     typedef typename Mf<T>::acc A4;
@@ -986,21 +997,22 @@ __device__ __forceinline__ void schur_pass(GAS T* __restrict__ Pr, GAS T* __rest
                 T br[TG], bi[TG];
 #pragma unroll
                 for (int u = 0; u < TG; ++u) {
-                    br[u] = ug + u < TN ? b0[16 * u] : T(0);
-                    bi[u] = (CPLX && ug + u < TN) ? b1[16 * u] : T(0);
+                    const bool ld0 = ug + u < TN && kstep_in_band<BAND>(ug + u, 0);
+                    br[u] = ld0 ? b0[16 * u] : T(0);
+                    bi[u] = (CPLX && ld0) ? b1[16 * u] : T(0);
                 }
 #pragma unroll
                 for (int s = 0; s < TK; ++s) {
                     T nbr[TG], nbi[TG];
 #pragma unroll
                     for (int u = 0; u < TG; ++u) {
-                        const bool ldn = s + 1 < TK && ug + u < TN;
+                        const bool ldn = s + 1 < TK && ug + u < TN && kstep_in_band<BAND>(ug + u, s + 1);
                         nbr[u] = ldn ? b0[4 * (s + 1) * LS + 16 * u] : T(0);
                         nbi[u] = (CPLX && ldn) ? b1[4 * (s + 1) * LS + 16 * u] : T(0);
                     }
 #pragma unroll
                     for (int u = 0; u < TG; ++u) {
-                        if (ug + u < TN) {
+                        if (ug + u < TN && kstep_in_band<BAND>(ug + u, s)) {
                             p1[u] = Mf<T>::mfma(ar[s], br[u], p1[u]);
                             if (CPLX) {
                                 p2[u] = Mf<T>::mfma(ai[s], bi[u], p2[u]);
@@ -1133,7 +1145,8 @@ __device__ __noinline__ int block_lu_tk(T* __restrict__ Pr_, T* __restrict__ Pi_
             }
             __syncthreads();
             lap(1);
-            schur_pass<CPLX, T, TK>(Pr, Pi, ps, ld, I, m, nU, kb0, nkb, L);
+            if (kb0 == 2) schur_pass<CPLX, T, TK, true>(Pr, Pi, ps, ld, I, m, nU, kb0, nkb, L);
+            else schur_pass<CPLX, T, TK>(Pr, Pi, ps, ld, I, m, nU, kb0, nkb, L);
             __syncthreads();
             lap(3);
         }
