@@ -10,6 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfpe_engine.so")
+LU_BENCH = os.path.join(OUT_DIR, "lu_bench")
 ARCH = os.environ.get("FPE_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -45,6 +46,10 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    # the factorisation microbenchmark (a measurement tool for profiles/, not part of the library)
+    tmp = LU_BENCH + ".tmp"
+    _run([HIPCC] + HIP_FLAGS + [os.path.join(CSRC, "lu_bench.hip"), "-o", tmp])
+    os.replace(tmp, LU_BENCH)
     return LIB
 
 
