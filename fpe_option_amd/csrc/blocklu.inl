@@ -15,6 +15,8 @@
 // inverse); E = a M + h K with D-scaled mass and stiffness is block diagonally
 // dominant for the step sizes Radau takes, which the parity tests exercise.
 #pragma once
+#include <type_traits>
+
 #include "device_common.hpp"
 
 namespace fpe {
@@ -39,6 +41,9 @@ struct LuLds {
     // Gauss-Jordan panel of GJ_PW pivot steps: column s of the panel right after
     // step s (re, im; at [s * GJ_US + row]), 1 / pivot, pivot rows, rows used so far
     double U[2][GJ_PW * GJ_US];
+    // U block (I, I+3) as its 7 diagonals (re, im; column j at [7 j + k - j + 3],
+    // zeros outside the block), staged for the last Schur pass of block row I
+    double UB[2][7 * MAX_BLOCK];
     double ip[2][GJ_PW];
     int pv[GJ_PW];
     unsigned long long umask;
@@ -793,6 +798,24 @@ __device__ __noinline__ void u_block_in(LuLds& L, int k, const T* sr, const T* s
     stage_in<CPLX, T, 1>(lds_block<T>(L, k), CPLX ? lds_block<T>(L, k + 1) : nullptr, uniform(gas(sr)),
                          CPLX ? uniform(gas(si)) : nullptr, ld, m);
 }
+// The band of U block (I, I+3) into UB (sr, si: its first column in panel I).
+template <bool CPLX, class T>
+__device__ __noinline__ void u_band_in(LuLds& L, const T* sr_, const T* si_, int ld, int m) {
+    m = uniform(m), ld = uniform(ld);
+    GAS const T* const sr = uniform(gas(sr_));
+    GAS const T* const si = CPLX ? uniform(gas(si_)) : nullptr;
+    T* ur = reinterpret_cast<T*>(L.UB[0]);
+    T* ui = reinterpret_cast<T*>(L.UB[1]);
+    for (int e = threadIdx.x; e < 7 * MAX_BLOCK; e += RNT) {
+        const int j = e / 7, k = j + e - 7 * j - 3;
+        const bool ok = j < m && k >= 0 && k < m;
+        const long long o = ok ? (long long)k * ld + j : 0;
+        const T xr = sr[o];
+        const T xi = CPLX ? si[o] : T(0);
+        ur[e] = ok ? xr : T(0);
+        if (CPLX) ui[e] = ok ? xi : T(0);
+    }
+}
 template <bool CPLX, class T>
 __device__ __noinline__ void pivot_block_out(LuLds& L, T* sr, T* si, int ld, int m) {
     m = uniform(m), ld = uniform(ld);
@@ -910,33 +933,34 @@ __device__ __forceinline__ void lower_multiply(GAS T* __restrict__ Pr, GAS T* __
     }
 }
 
-// The U block of the last Schur pass, U_{I,I+3}, still holds only its
-// assembled band (|k - j| <= 3): block (I, I+3) is updated by the steps p with
-// I <= p <= I - 1, i.e. by none. Column tile u (columns 16 u .. 16 u + 15)
-// then needs only the k-steps that meet rows 16 u - 3 .. 16 u + 18, 4 u - 1 to
-// 4 u + 4: six of the TK k-steps. The skipped k-steps' products are exact zeros,
-// so the result is the same as the full loop's.
-template <bool BAND>
-__device__ __forceinline__ constexpr bool kstep_in_band(int u, int s) {
-    return !BAND || (s >= 4 * u - 1 && s <= 4 * u + 4);
-}
+// The U block (I, I+3) still holds only its assembled band (|k - j| <= 3) when
+// step I uses it: block (I, I+3) is updated by the steps p with I <= p <= I - 1,
+// i.e. by none. It is staged compactly (UB: the 7 diagonals of each column) and
+// its update rides on the last dense pass of the block row, reusing that pass's
+// L strips; column tile u needs only the k-steps 4 u - 1 .. 4 u + 4, which meet
+// rows 16 u - 3 .. 16 u + 18. The skipped k-steps' products are exact zeros.
+__device__ __forceinline__ constexpr bool kstep_in_band(int u, int s) { return s >= 4 * u - 1 && s <= 4 * u + 4; }
 
 // A_JK -= L_JI U_IK for J = I+1..I+nU and K = I+1+kb0 .. I+kb0+nkb, with U_IK
 // staged in LDS (B[0] re, B[1] im in complex arithmetic, one block per pass;
-// B[kb] in real), compiled for TK k-steps like lower_multiply. An L strip is
-// loaded into registers once per strip and pass; C tiles are loaded before
-// their MFMA chains, which start from zero, and written once. Complex tiles one
-// at a time with three real product chains, real tiles four at a time (four
-// independent chains).
-template <bool CPLX, class T, int TK, bool BAND = false>
+// B[kb] in real), compiled for TK k-steps like lower_multiply; with BANDK also
+// K = I+3 from its staged band (UB). An L strip is loaded into registers once
+// per strip and pass; C tiles are loaded before their MFMA chains, which start
+// from zero, and written once. Complex tiles one at a time with three real
+// product chains, real tiles four at a time (four independent chains).
+// DENSE = false: only K = I+3, with U_{I,I+3} staged as a full block pair in
+// B[0], B[1] (the k-steps outside its band skipped).
+template <bool CPLX, class T, int TK, bool BANDK = false, bool DENSE = true>
 __device__ __forceinline__ void schur_pass(GAS T* __restrict__ Pr, GAS T* __restrict__ Pi, long long ps, int ld,
-                                            int I, int m, int nU, int kb0, int nkb, const LuLds& L) {
+                                           int I, int m, int nU, int kb0, int nkb, const LuLds& L) {
     typedef typename Mf<T>::acc A4;
     constexpr int TN = (TK + 3) / 4;
     constexpr int TG = CPLX ? 1 : 4;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, l4 = lane >> 4;
     const int M = nU * m, tm = (M + 15) >> 4;
+    const T* ubr = reinterpret_cast<const T*>(L.UB[0]);
+    const T* ubi = reinterpret_cast<const T*>(L.UB[1]);
     for (int t = wave; t < tm; t += RNW) {
         const int r0 = t << 4;
         T ar[TK], ai[TK];
@@ -965,14 +989,16 @@ __device__ __forceinline__ void schur_pass(GAS T* __restrict__ Pr, GAS T* __rest
             oc[j] = stacked_row(ps, ld, I, m, min(r, M - 1), dj[j]);
             fi[j] = min(r, M - 1) - (dj[j] - 1) * m;
         }
-        for (int kb = 0; kb < nkb; ++kb) {
-            const T* Br = lds_block<T>(L, CPLX ? 2 * kb : kb);
-            const T* Bi = lds_block<T>(L, CPLX ? 2 * kb + 1 : kb);
-            const int kk = kb0 + kb;
+        // one target block column K = I + 1 + kk. Operand mode: 0 dense U from the LDS
+        // block pair (Br, Bi); 1 U_{I,I+3} from UB (its band); 2 U_{I,I+3} as a full
+        // block in (Br, Bi), only the k-steps that meet its band
+        auto block = [&](int kk, const T* Br, const T* Bi, auto mode_tag) {
+            constexpr int MODE = decltype(mode_tag)::value;
+            constexpr bool BND = MODE != 0;
             // first update of a block: only the entries within 3 of its diagonal are loaded
-            int band[4];
+            int cb[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) band[j] = (I == 0 || dj[j] == 3 || kk == 2) ? fi[j] : -(1 << 20);
+            for (int j = 0; j < 4; ++j) cb[j] = (I == 0 || dj[j] == 3 || kk == 2) ? fi[j] : -(1 << 20);
 #pragma unroll
             for (int ug = 0; ug < TN; ug += TG) {
                 T cr[TG][4], ci[TG][4];
@@ -983,7 +1009,7 @@ __device__ __forceinline__ void schur_pass(GAS T* __restrict__ Pr, GAS T* __rest
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         const bool ok = ug + u < TN && vc[j] && c < m &&
-                                        (band[j] < 0 || (c - band[j] <= 3 && band[j] - c <= 3));
+                                        (cb[j] < 0 || (c - cb[j] <= 3 && cb[j] - c <= 3));
                         const long long o = oc[j] + (long long)(kk + 4 - dj[j]) * m + cc;
                         cr[u][j] = ok ? Pr[o] : T(0);
                         ci[u][j] = (CPLX && ok) ? Pi[o] : T(0);
@@ -991,28 +1017,40 @@ __device__ __forceinline__ void schur_pass(GAS T* __restrict__ Pr, GAS T* __rest
                 }
                 LAS const T* b0 = lds_base(Br + l4 * LS + (ug << 4) + l16);
                 LAS const T* b1 = lds_base(Bi + l4 * LS + (ug << 4) + l16);
+                // operand of k-step s for tile ug + u (zero outside the band / past the tiles)
+                auto opnd = [&](int u, int s, T& vr, T& vi) {
+                    if (!(ug + u < TN) || (BND && !kstep_in_band(ug + u, s))) {
+                        vr = vi = T(0);
+                    } else if constexpr (MODE == 1) {
+                        const int c = ((ug + u) << 4) + l16, dd = 4 * s + l4 - c + 3;
+                        const int e = c * 7 + min(max(dd, 0), 6);
+                        const bool in = dd >= 0 && dd <= 6;
+                        const T xr = ubr[e];
+                        const T xi = CPLX ? ubi[e] : T(0);
+                        vr = in ? xr : T(0);
+                        vi = (CPLX && in) ? xi : T(0);
+                    } else {
+                        vr = b0[4 * s * LS + 16 * u];
+                        vi = CPLX ? b1[4 * s * LS + 16 * u] : T(0);
+                    }
+                };
                 A4 p1[TG], p2[TG], p3[TG];
 #pragma unroll
                 for (int u = 0; u < TG; ++u) p1[u] = p2[u] = p3[u] = A4{0, 0, 0, 0};
                 T br[TG], bi[TG];
 #pragma unroll
-                for (int u = 0; u < TG; ++u) {
-                    const bool ld0 = ug + u < TN && kstep_in_band<BAND>(ug + u, 0);
-                    br[u] = ld0 ? b0[16 * u] : T(0);
-                    bi[u] = (CPLX && ld0) ? b1[16 * u] : T(0);
-                }
+                for (int u = 0; u < TG; ++u) opnd(u, 0, br[u], bi[u]);
 #pragma unroll
                 for (int s = 0; s < TK; ++s) {
                     T nbr[TG], nbi[TG];
 #pragma unroll
                     for (int u = 0; u < TG; ++u) {
-                        const bool ldn = s + 1 < TK && ug + u < TN && kstep_in_band<BAND>(ug + u, s + 1);
-                        nbr[u] = ldn ? b0[4 * (s + 1) * LS + 16 * u] : T(0);
-                        nbi[u] = (CPLX && ldn) ? b1[4 * (s + 1) * LS + 16 * u] : T(0);
+                        if (s + 1 < TK) opnd(u, s + 1, nbr[u], nbi[u]);
+                        else nbr[u] = nbi[u] = T(0);
                     }
 #pragma unroll
                     for (int u = 0; u < TG; ++u) {
-                        if (ug + u < TN && kstep_in_band<BAND>(ug + u, s)) {
+                        if (ug + u < TN && (!BND || kstep_in_band(ug + u, s))) {
                             p1[u] = Mf<T>::mfma(ar[s], br[u], p1[u]);
                             if (CPLX) {
                                 p2[u] = Mf<T>::mfma(ai[s], bi[u], p2[u]);
@@ -1042,6 +1080,15 @@ __device__ __forceinline__ void schur_pass(GAS T* __restrict__ Pr, GAS T* __rest
                     }
                 }
             }
+        };
+        if constexpr (DENSE) {
+            for (int kb = 0; kb < nkb; ++kb)
+                block(kb0 + kb, lds_block<T>(L, CPLX ? 2 * kb : kb), lds_block<T>(L, CPLX ? 2 * kb + 1 : kb),
+                      std::integral_constant<int, 0>());
+            if constexpr (BANDK) block(2, lds_block<T>(L, 0), lds_block<T>(L, 0), std::integral_constant<int, 1>());
+        } else {
+            // U_{I,I+3} staged as a full block pair in B[0], B[1]
+            block(2, lds_block<T>(L, 0), lds_block<T>(L, CPLX ? 1 : 0), std::integral_constant<int, 2>());
         }
     }
 }
@@ -1136,17 +1183,33 @@ __device__ __noinline__ int block_lu_tk(T* __restrict__ Pr_, T* __restrict__ Pi_
         lower_multiply<CPLX, T, TK>(Pr, Pi, ps, ld, I, m, nU, L);
         __syncthreads();
         lap(2);
-        for (int kb0 = 0; kb0 < nU; kb0 += KPASS) {
-            const int nkb = (nU - kb0) < KPASS ? (nU - kb0) : KPASS;
+        // U blocks I+1, I+2 through LDS, KPASS per pass; U_{I,I+3} (still its assembled
+        // band) from UB, riding on the last dense pass -- except in FP64 complex
+        // arithmetic at TK >= 15, where the merged pass's registers spill and the
+        // band update runs as a pass of its own
+        constexpr bool MERGE = !(CPLX && sizeof(T) == 8 && TK >= 15);
+        const int nD = nU < 3 ? nU : 2;
+        for (int kb0 = 0; kb0 < nD; kb0 += KPASS) {
+            const int nkb = (nD - kb0) < KPASS ? (nD - kb0) : KPASS;
+            const bool band = MERGE && kb0 + nkb == nD && nU == 3;
             for (int kb = 0; kb < nkb; ++kb) {
                 const int col = (4 + kb0 + kb) * m;
                 u_block_in<CPLX, T>(L, CPLX ? 2 * kb : kb, (const T*)(PIr + col), CPLX ? (const T*)(PIi + col) : nullptr,
                                     ld, m);
             }
+            if (band) u_band_in<CPLX, T>(L, (const T*)(PIr + 6 * m), CPLX ? (const T*)(PIi + 6 * m) : nullptr, ld, m);
             __syncthreads();
             lap(1);
-            if (kb0 == 2) schur_pass<CPLX, T, TK, true>(Pr, Pi, ps, ld, I, m, nU, kb0, nkb, L);
+            if (MERGE && band) schur_pass<CPLX, T, TK, MERGE>(Pr, Pi, ps, ld, I, m, nU, kb0, nkb, L);
             else schur_pass<CPLX, T, TK>(Pr, Pi, ps, ld, I, m, nU, kb0, nkb, L);
+            __syncthreads();
+            lap(3);
+        }
+        if (!MERGE && nU == 3) {
+            u_block_in<CPLX, T>(L, 0, (const T*)(PIr + 6 * m), CPLX ? (const T*)(PIi + 6 * m) : nullptr, ld, m);
+            __syncthreads();
+            lap(1);
+            schur_pass<CPLX, T, TK, false, false>(Pr, Pi, ps, ld, I, m, nU, 2, 0, L);
             __syncthreads();
             lap(3);
         }
