@@ -955,7 +955,9 @@ __device__ __forceinline__ void schur_pass(GAS T* __restrict__ Pr, GAS T* __rest
                                            int I, int m, int nU, int kb0, int nkb, const LuLds& L) {
     typedef typename Mf<T>::acc A4;
     constexpr int TN = (TK + 3) / 4;
-    constexpr int TG = CPLX ? 1 : 4;
+    // complex: two tiles (six chains) at a time where the registers allow it (FP32, or
+    // FP64 up to 8 k-steps); one tile otherwise (two spilled in the surrounding code)
+    constexpr int TG = CPLX ? ((sizeof(T) == 4 || TK <= 8) ? 2 : 1) : 4;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, l4 = lane >> 4;
     const int M = nU * m, tm = (M + 15) >> 4;
