@@ -137,6 +137,75 @@ int tuned_variant(int nsb, int nvb) {
     return best < 0 ? LU_GENERIC : g_tune[best].variant;
 }
 
+// Launch groups of the initial-condition kernels (budgets in ic.inl).
+struct IcGroup {
+    int start, count;  // a slice of the group order
+    int key;           // the group's launch key
+};
+
+// Sorts the instances into ord by launch key (ascending) and cost (largest
+// first) and cuts it into one group per key.
+template <class Key, class Cost>
+void ic_groups(int n, Key key, Cost cost, std::vector<int>& ord, std::vector<IcGroup>& groups) {
+    std::vector<int> k(n);
+    for (int s = 0; s < n; ++s) k[s] = key(s);
+    ord.resize(n);
+    for (int s = 0; s < n; ++s) ord[s] = s;
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+        if (k[a] != k[b]) return k[a] < k[b];
+        return cost(a) > cost(b);
+    });
+    groups.clear();
+    for (int i = 0; i < n;) {
+        int j = i;
+        while (j < n && k[ord[j]] == k[ord[i]]) ++j;
+        groups.push_back(IcGroup{i, j - i, k[ord[i]]});
+        i = j;
+    }
+}
+
+// k_eig launch class: the smallest LDS budget that holds A and Qt, else the largest.
+const long long kEigBudgets[4] = {EIG_LDS_S, EIG_LDS_HALF, EIG_LDS_THIRD, EIG_LDS_QUARTER};
+int eig_class(int n) {
+    int c = 0;
+    for (int b = 1; b < 4; ++b)
+        if (eig_lds_bytes(n) <= kEigBudgets[b]) c = b;
+    return c;
+}
+
+using AdmmKernel = void (*)(const Inst*, const int*, const double*, const double*, double*, InstOut*, long long);
+template <int NW, int MT, int TN>
+constexpr AdmmKernel admm_kernel() {
+    return k_admm<NW, MT, TN, (MT * TN <= ADMM_REG_TILES)>;
+}
+// [MT - 1][TN - 1]: 8-wave workgroups up to 3 strips per wave (n_sb <= 384), 4-wave ones up to 2
+const AdmmKernel kAdmm8[3][4] = {
+    {admm_kernel<8, 1, 1>(), admm_kernel<8, 1, 2>(), admm_kernel<8, 1, 3>(), admm_kernel<8, 1, 4>()},
+    {admm_kernel<8, 2, 1>(), admm_kernel<8, 2, 2>(), admm_kernel<8, 2, 3>(), admm_kernel<8, 2, 4>()},
+    {admm_kernel<8, 3, 1>(), admm_kernel<8, 3, 2>(), admm_kernel<8, 3, 3>(), admm_kernel<8, 3, 4>()}};
+const AdmmKernel kAdmm4[2][4] = {
+    {admm_kernel<4, 1, 1>(), admm_kernel<4, 1, 2>(), admm_kernel<4, 1, 3>(), admm_kernel<4, 1, 4>()},
+    {admm_kernel<4, 2, 1>(), admm_kernel<4, 2, 2>(), admm_kernel<4, 2, 3>(), admm_kernel<4, 2, 4>()}};
+
+// k_admm launch key: class (0: 8-wave workgroups with ADMM_LDS_MAX bytes; 1, 2:
+// 4-wave workgroups with a half or third of the CU's LDS) and the kernel's strip
+// and column-tile counts. Among the forms an instance fits (class 0 always; 1 and
+// 2 when its work matrices fit the budget and it has at most 8 strips), the one
+// with the most resident workgroups per CU (g_admm_occ, from the runtime's
+// occupancy of each kernel at its budget) wins; ties go to the lower class.
+const long long kAdmmBudgets[3] = {ADMM_LDS_MAX, IC_LDS_HALF, IC_LDS_THIRD};
+int g_admm_occ[3][3][4];  // [class][MT - 1][TN - 1] workgroups per CU (class 0: MT up to 3, else 2)
+int admm_key(int ns, int nv) {
+    const int tn = admm_tn(nv);
+    int best = 0, best_occ = g_admm_occ[0][admm_mt(ns, 8) - 1][tn - 1];
+    if (((ns + 15) >> 4) <= 8)
+        for (int c = 1; c < 3; ++c)
+            if (admm_lds_bytes(ns, nv, false) <= kAdmmBudgets[c] && g_admm_occ[c][admm_mt(ns, 4) - 1][tn - 1] > best_occ)
+                best = c, best_occ = g_admm_occ[c][admm_mt(ns, 4) - 1][tn - 1];
+    const int mt = admm_mt(ns, best == 0 ? 8 : 4);
+    return best * 16 + (mt - 1) * 4 + (tn - 1);
+}
+
 bool dev_init(std::string& err) {
     if (g_dev.init) return true;
     int n = 0;
@@ -147,8 +216,23 @@ bool dev_init(std::string& err) {
     }
     HIP_OK(hipSetDevice(0));
     HIP_OK(hipDeviceGetAttribute(&g_dev.n_cu, hipDeviceAttributeMultiprocessorCount, 0));
-    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_admm), hipFuncAttributeMaxDynamicSharedMemorySize,
-                               int(ADMM_LDS_MAX)));
+    for (const auto& row : kAdmm8)
+        for (AdmmKernel k : row)
+            HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       int(ADMM_LDS_MAX)));
+    for (const auto& row : kAdmm4)
+        for (AdmmKernel k : row)
+            HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       int(IC_LDS_HALF)));
+    for (int c = 0; c < 3; ++c)
+        for (int mt = 1; mt <= (c == 0 ? 3 : 2); ++mt)
+            for (int tn = 1; tn <= 4; ++tn) {
+                const AdmmKernel k = c == 0 ? kAdmm8[mt - 1][tn - 1] : kAdmm4[mt - 1][tn - 1];
+                int nb = 0;
+                HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(k),
+                                                                    (c == 0 ? 8 : 4) * 64, size_t(kAdmmBudgets[c])));
+                g_admm_occ[c][mt - 1][tn - 1] = nb;
+            }
     HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_eig), hipFuncAttributeMaxDynamicSharedMemorySize,
                                int(EIG_LDS_S)));
     HIP_OK(hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking));
@@ -170,6 +254,7 @@ bool upload(void* dst, const std::vector<T>& h, std::string& err) {
 }
 
 size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
 
 }  // namespace
 
@@ -297,6 +382,21 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
         return ca > cb;
     });
 
+    // initial-condition launch groups: the s and v eigenproblems (A and Qt in
+    // LDS when the class budget holds both), then ADMM
+    std::vector<int> ord_e0, ord_e1, ord_a;
+    std::vector<IcGroup> grp_e0, grp_e1, grp_a;
+    ic_groups(n, [&](int s) { return eig_class(hi[s].nsb); },
+              [&](int s) { return double(hi[s].nsb) * hi[s].nsb * hi[s].nsb; }, ord_e0, grp_e0);
+    ic_groups(n, [&](int s) { return eig_class(hi[s].nvb); },
+              [&](int s) { return double(hi[s].nvb) * hi[s].nvb * hi[s].nvb; }, ord_e1, grp_e1);
+    ic_groups(n, [&](int s) { return admm_key(hi[s].nsb, hi[s].nvb); },
+              [&](int s) { return double(hi[s].nsb) * hi[s].nvb * (hi[s].nsb + hi[s].nvb); }, ord_a, grp_a);
+    const long long oo_e0 = n, oo_e1 = 2LL * n, oo_a = 3LL * n;  // offsets of the group orders after the Radau order
+    order.insert(order.end(), ord_e0.begin(), ord_e0.end());
+    order.insert(order.end(), ord_e1.begin(), ord_e1.end());
+    order.insert(order.end(), ord_a.begin(), ord_a.end());
+
     std::vector<PriceReq> hr(reqs.size());
     for (size_t k = 0; k < reqs.size(); ++k) {
         hr[k].inst = reqs[k].solve, hr[k].option_type = reqs[k].option_type;
@@ -415,12 +515,19 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
     if (!given) {
         hipLaunchKernelGGL(k_ic_rhs, dim3(n), dim3(NT), 0, st, dI, D.order, D.qd,
                            D.bv, D.bc, D.vc);
-        hipLaunchKernelGGL(k_eig, dim3(n), dim3(ENT), EIG_LDS_S, st, dI, D.order, 0, EIG_LDS_S,
-                           D.fc, D.eg);
-        hipLaunchKernelGGL(k_eig, dim3(n), dim3(ENT), EIG_LDS_V, st, dI, D.order, 1, EIG_LDS_V,
-                           D.fc, D.eg);
-        hipLaunchKernelGGL(k_admm, dim3(n), dim3(ANT), ADMM_LDS_MAX, st, dI, D.order, D.st,
-                           D.eg, D.vc, D.out);
+        for (const IcGroup& g : grp_e0)
+            hipLaunchKernelGGL(k_eig, dim3(g.count), dim3(ENT), kEigBudgets[g.key], st, dI, D.order + oo_e0 + g.start,
+                               0, kEigBudgets[g.key], D.fc, D.eg);
+        for (const IcGroup& g : grp_e1)
+            hipLaunchKernelGGL(k_eig, dim3(g.count), dim3(ENT), kEigBudgets[g.key], st, dI, D.order + oo_e1 + g.start,
+                               1, kEigBudgets[g.key], D.fc, D.eg);
+        for (const IcGroup& g : grp_a) {
+            const int cls = g.key / 16, mt = (g.key / 4) % 4 + 1, tn = g.key % 4 + 1;
+            const long long lds = kAdmmBudgets[cls];
+            const AdmmKernel k = cls == 0 ? kAdmm8[mt - 1][tn - 1] : kAdmm4[mt - 1][tn - 1];
+            hipLaunchKernelGGL(k, dim3(g.count), dim3((cls == 0 ? 8 : 4) * 64), lds, st, dI,
+                               D.order + oo_a + g.start, D.st, D.eg, D.vc, D.out, lds);
+        }
         HIP_OK(hipGetLastError());
     }
     HIP_OK(hipEventRecord(G.ev[2], st));

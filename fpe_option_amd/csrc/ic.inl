@@ -103,8 +103,9 @@ __global__ void __launch_bounds__(NT) k_ic_rhs(const Inst* __restrict__ inst, co
 // operands live.
 constexpr int ENT = 512;  // k_eig workgroup: 8 waves
 constexpr int ENW = ENT / 64;
-constexpr long long EIG_LDS_S = 150 * 1024;                          // axis 0 launch
-constexpr long long EIG_LDS_V = 2LL * MAX_BLOCK * (MAX_BLOCK + 1) * 8;  // axis 1: A and Qt of n_vb <= 64
+constexpr long long EIG_LDS_S = 150 * 1024;  // the largest dynamic LDS of a k_eig launch
+// Dynamic LDS of k_eig for an n x n problem with A and Qt both in LDS.
+__host__ __device__ inline long long eig_lds_bytes(int n) { return 16LL * n * (n + 1); }
 
 __global__ void __launch_bounds__(ENT) k_eig(const Inst* __restrict__ inst, const int* __restrict__ order, int ax,
                                              long long lds_bytes, const double* __restrict__ FC,
@@ -222,32 +223,51 @@ __global__ void __launch_bounds__(ENT) k_eig(const Inst* __restrict__ inst, cons
     }
 }
 
-// C = A B (N <= 64) with a K-loop, v_mfma_f64_16x16x4f64; operands typically in
-// LDS. A wave owns a 16-row strip across all (at most 4) column tiles: each
-// k-step feeds one A fragment to four independent accumulator chains.
+// C = A B (M <= 16 NW MT rows, N <= 16 TN columns) with a K-loop on
+// v_mfma_f64_16x16x4f64; operands typically in LDS. A wave owns the 16-row
+// strips wave, wave + NW, ... (at most MT of them) across all TN column tiles:
+// each k-step feeds one A fragment to TN independent accumulator chains.
+// Operand reads are branch-free: indices past the matrix are clamped to its
+// last row or column and the value is replaced by zero, so a k-step's TN
+// B reads issue together. Padded k-steps add exact zeros (0 * 0).
 // AP > 0 (an A operand read from memory: the s eigenvectors when they do not
 // fit in LDS): A fragments are loaded a chunk of AP k-steps ahead of their use,
 // so the operand waits once per chunk, behind the previous chunk's MFMAs. The
-// k order is the same either way, so results are identical. C(r, c, value)
-// receives each output once. NW: waves in the workgroup.
-template <int NW, int AP = 0, class FA, class FB, class FC>
-__device__ __forceinline__ void mfma_gemm_k(int M, int N, int K, FA A, FB B, FC C) {
+// k order is the same either way, so results are identical.
+// T(it, r0, acc) receives strip it (rows r0 .. r0 + 15) with
+// acc[u][j] = C[r0 + lane/16 + 4 j][16 u + lane%16].
+template <int NW, int MT, int TN, int AP = 0, class FA, class FB, class FT>
+__device__ __forceinline__ void mfma_gemm_t(int M, int N, int K, FA A, FB B, FT T) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int tm = (M + 15) >> 4, tn = (N + 15) >> 4;
+    const int tm = (M + 15) >> 4;
     const int l16 = lane & 15, l4 = lane >> 4;
-    for (int t = wave; t < tm; t += NW) {
-        const int r0 = t << 4, ra = r0 + l16;
-        const bool va = ra < M;
-        d4 acc[4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
-        auto kstep = [&](double a, int k) {
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                if (u < tn) {
-                    const int cb = (u << 4) + l16;
-                    const double b = (cb < N && k < K) ? B(k, cb) : 0.0;
-                    acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[u], 0, 0, 0);
-                }
+    for (int it = 0; it < MT; ++it) {
+        const int t = wave + it * NW;
+        if (t >= tm) break;  // wave-uniform
+        const int r0 = t << 4;
+        const bool va = r0 + l16 < M;
+        const int ra = va ? r0 + l16 : M - 1;
+        d4 acc[TN];
+#pragma unroll
+        for (int u = 0; u < TN; ++u) acc[u] = d4{0.0, 0.0, 0.0, 0.0};
+        auto aload = [&](int k) {
+            const bool vk = k < K;
+            const double x = A(ra, vk ? k : K - 1);
+            return (va && vk) ? x : 0.0;
+        };
+        auto kstep = [&](double a, int k) {
+            const bool vk = k < K;
+            const int kc = vk ? k : K - 1;
+            double b[TN];
+#pragma unroll
+            for (int u = 0; u < TN; ++u) {
+                const int cb = (u << 4) + l16;
+                const double x = B(kc, cb < N ? cb : N - 1);
+                b[u] = (vk && cb < N) ? x : 0.0;
             }
+#pragma unroll
+            for (int u = 0; u < TN; ++u) acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[u], acc[u], 0, 0, 0);
         };
         int k0 = 0;
         if constexpr (AP > 0) {
@@ -256,14 +276,14 @@ __device__ __forceinline__ void mfma_gemm_k(int M, int N, int K, FA A, FB B, FC 
             double a[AP];
             if (kfull > 0) {
 #pragma unroll
-                for (int j = 0; j < AP; ++j) a[j] = va ? A(ra, 4 * j + l4) : 0.0;
+                for (int j = 0; j < AP; ++j) a[j] = aload(4 * j + l4);
             }
 #pragma unroll 1
             for (; k0 < kfull; k0 += 4 * AP) {
                 double an[AP];
                 const bool more = k0 + 4 * AP < kfull;
 #pragma unroll
-                for (int j = 0; j < AP; ++j) an[j] = (va && more) ? A(ra, k0 + 4 * AP + 4 * j + l4) : 0.0;
+                for (int j = 0; j < AP; ++j) an[j] = aload(more ? k0 + 4 * AP + 4 * j + l4 : 4 * j + l4);
 #pragma unroll
                 for (int j = 0; j < AP; ++j) kstep(a[j], k0 + 4 * j + l4);
 #pragma unroll
@@ -271,70 +291,92 @@ __device__ __forceinline__ void mfma_gemm_k(int M, int N, int K, FA A, FB B, FC 
             }
         }
 #pragma unroll 2
-        for (; k0 < K; k0 += 4) {
-            const int k = k0 + l4;
-            kstep((va && k < K) ? A(ra, k) : 0.0, k);
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int cb = (u << 4) + l16;
-            if (u < tn && cb < N) {
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int r = r0 + l4 + 4 * j;
-                    if (r < M) C(r, cb, acc[u][j]);
-                }
-            }
-        }
+        for (; k0 < K; k0 += 4) kstep(aload(k0 + l4), k0 + l4);
+        T(it, r0, acc);
     }
 }
 
 // Bytes of dynamic LDS k_admm needs for an ns x nv instance, with (qs_in_lds)
 // or without the ns x ns eigenvector matrix.
 __host__ __device__ inline long long admm_lds_bytes(int ns, int nv, bool qs_in_lds) {
-    return 8LL * ((long long)nv * (nv + 1) + 2LL * ns * (nv + 1) + (qs_in_lds ? (long long)ns * (ns + 1) : 0));
+    return 8LL * ((long long)nv * (nv + 1) + 2LL * ns * (nv + 1) + ns + nv +
+                  (qs_in_lds ? (long long)ns * (ns + 1) : 0));
 }
-constexpr long long ADMM_LDS_MAX = 150 * 1024;
+constexpr long long ADMM_LDS_MAX = 159 * 1024;  // with the 8-wave kernel's 320 B of static LDS: under 160 KB
+
+// Dynamic LDS budgets of the launch groups of k_eig and k_admm. A CU holds
+// 160 KB of LDS; an instance whose dynamic LDS need fits a half, third or
+// quarter of it (less the kernel's static LDS: 4 KB in k_eig, under 0.5 KB in
+// k_admm) runs in a launch whose workgroups take that much, so that many of
+// them share a CU.
+constexpr long long IC_LDS_HALF = 78 * 1024, IC_LDS_THIRD = 52 * 1024;  // k_admm
+constexpr long long EIG_LDS_HALF = 76 * 1024, EIG_LDS_THIRD = 49 * 1024, EIG_LDS_QUARTER = 36 * 1024;
+
+// Strip and column-tile counts of an ns x nv ADMM instance in a workgroup of
+// nw waves: MT = ceil(ceil(ns / 16) / nw), TN = ceil(nv / 16). The ADMM state
+// stays in registers when MT * TN <= ADMM_REG_TILES.
+constexpr int ADMM_REG_TILES = 8;
+__host__ __device__ inline int admm_mt(int ns, int nw) { return (((ns + 15) >> 4) + nw - 1) / nw; }
+__host__ __device__ inline int admm_tn(int nv) { return (nv + 15) >> 4; }
 
 // ADMM NNLS (osqp.mojo:93-157): rho 0.1, sigma 1e-6, lambda 1e-6, eps 1e-8,
 // at most 50000 iterations; then q0 = D z, clip at 0, normalise by m.q0
 // (initial_cond.mojo:119-129). The four products per iteration run on MFMA
-// with Qv and the two ns x nv work matrices in LDS (Qs too when it fits).
-// grid: n_inst; dynamic LDS: ADMM_LDS_MAX.
-constexpr int ANT = 512;  // k_admm workgroup: 8 waves, 2 per SIMD
-constexpr int ANW = ANT / 64;
-
-__global__ void __launch_bounds__(ANT) k_admm(const Inst* __restrict__ inst, const int* __restrict__ order, const double* __restrict__ ST,
-                                             const double* __restrict__ EG, double* __restrict__ VC,
-                                             InstOut* __restrict__ out) {
+// with Qv, the two ns x nv work matrices and the squared eigenvalues in LDS
+// (Qs too when it fits in lds_budget, the launch's dynamic LDS size).
+// The iteration's state is one value per entry, w = x + u before the
+// projection: z = max(w, 0) and u = w - z = min(w, 0) exactly, so
+//   w' = x' + min(w, 0),  z' = max(w', 0),  u' = min(w', 0)
+// reproduce the z and u updates bit for bit. Each thread keeps w and q for the
+// entries the last product's epilogue gives it in registers (REG, when the
+// wave's MT x TN tiles allow) or in the instance's z and q vectors.
+// grid: the instances order[0 .. grid - 1]; ANW waves per workgroup; every
+// instance of a launch has admm_mt(ns, ANW) <= MT and admm_tn(nv) == TN.
+// Launches are grouped by LDS need: instances whose work matrices fit a third
+// (or half) of the CU's LDS run as 4-wave workgroups, three (two) per CU, so
+// small grids, whose iterations are latency chains, overlap each other; the
+// largest run as 8-wave workgroups with up to ADMM_LDS_MAX bytes, one per CU.
+// (The host picks, per instance, the form with the most resident workgroups per
+// CU by the occupancy the runtime reports for each kernel.)
+// Every product is computed in the same order in every form; only the
+// residual norms' and m.q0's partial sums are split over threads differently.
+template <int ANW, int MT, int TN, bool REG>
+__global__ void __launch_bounds__(ANW * 64) k_admm(const Inst* __restrict__ inst, const int* __restrict__ order,
+                                                   const double* __restrict__ ST, const double* __restrict__ EG,
+                                                   double* __restrict__ VC, InstOut* __restrict__ out,
+                                                   long long lds_budget) {
+    constexpr int ANT = ANW * 64;
     extern __shared__ double dyn[];
-    const Inst& in = inst[order[blockIdx.x]];  // largest instances first
+    const Inst& in = inst[order[blockIdx.x]];
     const int ns = in.nsb, nv = in.nvb, N = ns * nv;
     EigView E(const_cast<double*>(EG) + in.o_eig, ns, nv);
     double* base = VC + in.o_vec;
     const double* Di = base + (long long)V_DINV * N;
     const double* bt = base + (long long)V_BT * N;
     const double* mw = base + (long long)V_MW * N;
-    double* z = base + (long long)V_ADMM_Z * N;
-    double* u = base + (long long)V_ADMM_U * N;
-    double* q = base + (long long)V_ADMM_Q * N;
+    double* wg = base + (long long)V_ADMM_Z * N;  // w when not in registers
+    double* qg = base + (long long)V_ADMM_Q * N;  // q when not in registers
     double* q0 = base + (long long)V_Q0 * N;
     const double* Ms = ST + in.o_st;
     const double rho = 0.1, sigma = 1e-6, lam = 1e-6, eps_abs = 1e-8, eps_rel = 1e-8;
     const double alpha = lam + sigma + rho;
     __shared__ double red[5 * ANW];
-    // LDS: Qv (nv x nv+1), T and R (ns x nv+1), Qs (ns x ns+1) when it fits
+    // LDS: Qv (nv x nv+1), T and R (ns x nv+1), ls^2 (ns), lv^2 (nv), Qs (ns x ns+1) when it fits
     const int lv = nv + 1, lsd = ns + 1;
     double* sQv = dyn;
     double* sT = sQv + nv * lv;
     double* sR = sT + ns * lv;
-    const bool qs_lds = admm_lds_bytes(ns, nv, true) <= ADMM_LDS_MAX;
-    double* sQs = sR + ns * lv;
+    double* sL2 = sR + ns * lv;
+    double* sV2 = sL2 + ns;
+    const bool qs_lds = admm_lds_bytes(ns, nv, true) <= lds_budget;
+    double* sQs = sV2 + nv;
     for (int e = threadIdx.x; e < nv * nv; e += ANT) sQv[(e / nv) * lv + e % nv] = E.Qv[e];
     if (qs_lds)
         for (int e = threadIdx.x; e < ns * ns; e += ANT) sQs[(e / ns) * lsd + e % ns] = E.Qs[e];
+    for (int a = threadIdx.x; a < ns; a += ANT) sL2[a] = E.ls[a] * E.ls[a];
+    for (int b = threadIdx.x; b < nv; b += ANT) sV2[b] = E.lv[b] * E.lv[b];
     auto Qs = [&](int I, int a) -> double { return qs_lds ? sQs[I * lsd + a] : E.Qs[(long long)I * ns + a]; };
-    // q = A^T b~ (A symmetric)
+    // q = A^T b~ (A symmetric); it is also the first right-hand side (x = z = u = 0)
     for (int r = threadIdx.x; r < N; r += ANT) {
         const int I = r / nv, i = r - I * nv;
         double acc = 0.0;
@@ -347,70 +389,163 @@ __global__ void __launch_bounds__(ANT) k_admm(const Inst* __restrict__ inst, con
                 acc += Ms[(long long)((dJ + 3) * 7 + dj + 3) * N + r] * bt[J * nv + jj];
             }
         }
-        q[r] = acc;
-        z[r] = 0.0, u[r] = 0.0;
-        sR[I * lv + i] = acc;  // first right-hand side: x = z = u = 0
+        if (!REG) {
+            qg[r] = acc;
+            wg[r] = 0.0;
+        }
+        sR[I * lv + i] = acc;
     }
     __syncthreads();
-    int it = 0;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, l16 = lane & 15, l4 = lane >> 4;
+    // this thread's entries of the last product: strip it, tile u, row j
+    auto ent_row = [&](int it, int j) { return ((wave + it * ANW) << 4) + l4 + 4 * j; };
+    auto ent_col = [&](int u) { return (u << 4) + l16; };
+    double wr[REG ? MT : 1][REG ? TN : 1][4], qr[REG ? MT : 1][REG ? TN : 1][4];
+    if constexpr (REG) {
+#pragma unroll
+        for (int it = 0; it < MT; ++it)
+#pragma unroll
+            for (int u = 0; u < TN; ++u)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int I = ent_row(it, j), i = ent_col(u);
+                    qr[it][u][j] = (I < ns && i < nv) ? sR[I * lv + i] : 0.0;
+                    wr[it][u][j] = 0.0;
+                }
+    }
+    int iters = 0;
     const int max_iter = 50000;
-    for (; it < max_iter;) {
+    for (; iters < max_iter;) {
         // T = Qs^T R
-        mfma_gemm_k<ANW, 4>(ns, nv, ns, [&](int a, int I) { return Qs(I, a); }, [&](int I, int i) { return sR[I * lv + i]; },
-                    [&](int a, int i, double v) { sT[a * lv + i] = v; });
+        mfma_gemm_t<ANW, MT, TN, 4>(ns, nv, ns, [&](int a, int I) { return Qs(I, a); },
+                                    [&](int I, int i) { return sR[I * lv + i]; },
+                                    [&](int, int r0, d4* acc) {
+#pragma unroll
+                                        for (int u = 0; u < TN; ++u)
+#pragma unroll
+                                            for (int j = 0; j < 4; ++j) {
+                                                const int a = r0 + l4 + 4 * j, i = ent_col(u);
+                                                if (a < ns && i < nv) sT[a * lv + i] = acc[u][j];
+                                            }
+                                    });
         __syncthreads();
         // R = (T Qv) ./ (ls^2 lv^2 + alpha)
-        mfma_gemm_k<ANW>(ns, nv, nv, [&](int a, int i) { return sT[a * lv + i]; },
-                    [&](int i, int b) { return sQv[i * lv + b]; },
-                    [&](int a, int b, double v) {
-                        const double la = E.ls[a] * E.ls[a], lb = E.lv[b] * E.lv[b];
-                        sR[a * lv + b] = v / (la * lb + alpha);
-                    });
+        mfma_gemm_t<ANW, MT, TN>(ns, nv, nv, [&](int a, int i) { return sT[a * lv + i]; },
+                                 [&](int i, int b) { return sQv[i * lv + b]; },
+                                 [&](int, int r0, d4* acc) {
+#pragma unroll
+                                     for (int u = 0; u < TN; ++u)
+#pragma unroll
+                                         for (int j = 0; j < 4; ++j) {
+                                             const int a = r0 + l4 + 4 * j, b = ent_col(u);
+                                             if (a < ns && b < nv) sR[a * lv + b] = acc[u][j] / (sL2[a] * sV2[b] + alpha);
+                                         }
+                                 });
         __syncthreads();
         // T = Qs R
-        mfma_gemm_k<ANW, 4>(ns, nv, ns, [&](int I, int a) { return Qs(I, a); }, [&](int a, int b) { return sR[a * lv + b]; },
-                    [&](int I, int b, double v) { sT[I * lv + b] = v; });
+        mfma_gemm_t<ANW, MT, TN, 4>(ns, nv, ns, [&](int I, int a) { return Qs(I, a); },
+                                    [&](int a, int b) { return sR[a * lv + b]; },
+                                    [&](int, int r0, d4* acc) {
+#pragma unroll
+                                        for (int u = 0; u < TN; ++u)
+#pragma unroll
+                                            for (int j = 0; j < 4; ++j) {
+                                                const int I = r0 + l4 + 4 * j, b = ent_col(u);
+                                                if (I < ns && b < nv) sT[I * lv + b] = acc[u][j];
+                                            }
+                                    });
         __syncthreads();
-        // x = T Qv^T, then the z and u updates, the five norms and the next
-        // right-hand side q + rho (z - u) + sigma x (R is free: GEMM 3 read it
-        // before the last barrier; x itself is never stored)
+        // x = T Qv^T, then the w update, the five norms and the next right-hand
+        // side q + rho (z - u) + sigma x (R is free: GEMM 3 read it before the
+        // last barrier; x itself is never stored)
         double nr[5] = {0, 0, 0, 0, 0};
-        mfma_gemm_k<ANW>(ns, nv, nv, [&](int I, int b) { return sT[I * lv + b]; },
-                    [&](int b, int i) { return sQv[i * lv + b]; },
-                    [&](int I, int i, double xv) {
-                        const int e = I * nv + i;
-                        const double zo = z[e], uo = u[e], qe = q[e];
-                        double zn = xv + uo;
-                        if (zn < 0.0) zn = 0.0;
-                        z[e] = zn;
-                        const double un = uo + xv - zn;
-                        u[e] = un;
-                        sR[I * lv + i] = qe + rho * (zn - un) + sigma * xv;
-                        const double pv = xv - zn, dv = rho * (zn - zo);
-                        nr[0] += pv * pv;
-                        nr[1] += dv * dv;
-                        nr[2] += zn * zn;
-                        nr[3] += xv * xv;
-                        nr[4] += un * un;
-                    });
+        mfma_gemm_t<ANW, MT, TN>(ns, nv, nv, [&](int I, int b) { return sT[I * lv + b]; },
+                                 [&](int b, int i) { return sQv[i * lv + b]; },
+                                 [&](int it, int r0, d4* acc) {
+                                     double wo[TN][4], qe[TN][4];
+#pragma unroll
+                                     for (int u = 0; u < TN; ++u)
+#pragma unroll
+                                         for (int j = 0; j < 4; ++j) {
+                                             if constexpr (REG) {
+                                                 wo[u][j] = wr[it][u][j], qe[u][j] = qr[it][u][j];
+                                             } else {  // every load of the strip before any store
+                                                 const int I = r0 + l4 + 4 * j, i = ent_col(u);
+                                                 const int e = (I < ns ? I : ns - 1) * nv + (i < nv ? i : nv - 1);
+                                                 wo[u][j] = wg[e], qe[u][j] = qg[e];
+                                             }
+                                         }
+#pragma unroll
+                                     for (int u = 0; u < TN; ++u)
+#pragma unroll
+                                         for (int j = 0; j < 4; ++j) {
+                                             const int I = r0 + l4 + 4 * j, i = ent_col(u);
+                                             if (I < ns && i < nv) {
+                                                 const double xv = acc[u][j], w = wo[u][j];
+                                                 const double zo = w < 0.0 ? 0.0 : w, uo = w < 0.0 ? w : 0.0;
+                                                 const double wn = xv + uo;
+                                                 const double zn = wn < 0.0 ? 0.0 : wn, un = wn < 0.0 ? wn : 0.0;
+                                                 if constexpr (REG) wr[it][u][j] = wn;
+                                                 else wg[I * nv + i] = wn;
+                                                 sR[I * lv + i] = qe[u][j] + rho * (zn - un) + sigma * xv;
+                                                 const double pv = xv - zn, dv = rho * (zn - zo);
+                                                 nr[0] += pv * pv;
+                                                 nr[1] += dv * dv;
+                                                 nr[2] += zn * zn;
+                                                 nr[3] += xv * xv;
+                                                 nr[4] += un * un;
+                                             }
+                                         }
+                                 });
         block_sum_n<5, ANW>(nr, red);  // its first barrier also orders the R writes before GEMM 1
-        ++it;
+        ++iters;
         const double ep = eps_abs + eps_rel * sqrt(fmax(nr[2], nr[3]));
         const double ed = eps_abs + eps_rel * rho * sqrt(nr[4]);
         if (sqrt(nr[0]) < ep && sqrt(nr[1]) < ed) break;
     }
     __syncthreads();
+    // q0 = max(D z, 0) / m.q0 with z = max(w, 0)
     double md = 0.0;
-    for (int r = threadIdx.x; r < N; r += ANT) {
-        double v = Di[r] * z[r];
-        if (v < 0.0) v = 0.0;
-        q0[r] = v;
-        md += mw[r] * v;
+    if constexpr (REG) {
+#pragma unroll
+        for (int it = 0; it < MT; ++it)
+#pragma unroll
+            for (int u = 0; u < TN; ++u)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int I = ent_row(it, j), i = ent_col(u);
+                    if (I < ns && i < nv) {
+                        const int e = I * nv + i;
+                        const double w = wr[it][u][j];
+                        double v = Di[e] * (w < 0.0 ? 0.0 : w);
+                        if (v < 0.0) v = 0.0;
+                        wr[it][u][j] = v;
+                        md += mw[e] * v;
+                    }
+                }
+        md = block_sum_w<ANW>(md, red);
+#pragma unroll
+        for (int it = 0; it < MT; ++it)
+#pragma unroll
+            for (int u = 0; u < TN; ++u)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int I = ent_row(it, j), i = ent_col(u);
+                    if (I < ns && i < nv) q0[I * nv + i] = md > 0.0 ? wr[it][u][j] / md : wr[it][u][j];
+                }
+    } else {
+        for (int r = threadIdx.x; r < N; r += ANT) {
+            const double w = wg[r];
+            double v = Di[r] * (w < 0.0 ? 0.0 : w);
+            if (v < 0.0) v = 0.0;
+            q0[r] = v;
+            md += mw[r] * v;
+        }
+        md = block_sum_w<ANW>(md, red);
+        if (md > 0.0)
+            for (int r = threadIdx.x; r < N; r += ANT) q0[r] = q0[r] / md;
     }
-    md = block_sum_w<ANW>(md, red);
-    if (md > 0.0)
-        for (int r = threadIdx.x; r < N; r += ANT) q0[r] = q0[r] / md;
-    if (threadIdx.x == 0) out[order[blockIdx.x]].n_admm = it;
+    if (threadIdx.x == 0) out[order[blockIdx.x]].n_admm = iters;
 }
 
 }  // namespace fpe
