@@ -167,10 +167,10 @@ def vec(ptr, n):
 
 def batch_options(n):
     arr = (FpeBatchOption * n)()
-    d = FpeBatchOption()
-    lib().fpe_batch_option_default(C.byref(d))
-    for i in range(n):
-        C.memmove(C.byref(arr[i]), C.byref(d), C.sizeof(d))
+    d = (FpeBatchOption * 1)()
+    lib().fpe_batch_option_default(C.byref(d[0]))
+    if n > 0:
+        np.ctypeslib.as_array(arr)[:] = np.ctypeslib.as_array(d)[0]
     return arr
 
 
@@ -194,7 +194,10 @@ def price_batch_raw(arr, want_info=False, want_qT=False):
     if rc != 0:
         raise RuntimeError(f"fpe_batch_price failed ({rc}): {last_error()}")
     out = (prices, status, st.as_dict())
-    recs = [info[i].as_dict() for i in range(n)]
+    if not (want_info or want_qT):
+        return out
+    names = [f for f, _ in FpeSolveInfo._fields_]
+    recs = [dict(zip(names, row)) for row in np.ctypeslib.as_array(info)[:n].tolist()]
     if want_info:
         out += (recs,)
     if want_qT:

@@ -333,16 +333,27 @@ _BATCH_FIELDS = ("kappa", "theta", "sigma", "rho", "r", "T", "S0", "V0", "s_min"
 
 
 def _batch_array(columns):
+    """FpeBatchOption array from scalar or per-option columns, written through a
+    numpy view of the ctypes array (one assignment per field)."""
     n = max([np.size(v) for v in columns.values()] + [1])
     arr = L.batch_options(n)
+    view = np.ctypeslib.as_array(arr)
     for k, v in columns.items():
         if k not in _BATCH_FIELDS:
             raise KeyError(k)
         if k == "option_type":
-            v = [_option_type_int(x) for x in np.atleast_1d(v)] if not np.isscalar(v) else _option_type_int(v)
-        a = np.broadcast_to(np.asarray(v), (n,))
-        for i in range(n):
-            setattr(arr[i], k, a[i].item())
+            a = np.asarray(v)
+            if a.dtype.kind in "iu":
+                if a.size and (a.min() < 0 or a.max() > 9):
+                    raise ValueError(f"option_type must be 0-9, got {a.min() if a.min() < 0 else a.max()}")
+            else:
+                v = [_option_type_int(x) for x in np.atleast_1d(v)] if not np.isscalar(v) else _option_type_int(v)
+        a = np.asarray(v)
+        if view.dtype[k].kind == "i" and a.dtype.kind not in "iub":
+            raise TypeError(f"{k} must be an integer, got {a.dtype}")
+        if a.dtype.kind not in "iubf":
+            raise TypeError(f"{k} must be numeric, got {a.dtype}")
+        view[k] = np.broadcast_to(a, (n,))
     return arr
 
 
@@ -354,15 +365,11 @@ def price_batch(return_stats: bool = False, return_info: bool = False, return_qT
     solver records (return_info: status, solve counters, grid) and a list of
     terminal states q(T) (return_qT; None for in-options)."""
     arr = _batch_array(columns)
-    prices, status, stats, info, *q = L.price_batch_raw(arr, want_info=True, want_qT=return_qT)
+    prices, status, stats, *rest = L.price_batch_raw(arr, want_info=return_info, want_qT=return_qT)
     out = (prices, status)
     if return_stats:
         out += (stats,)
-    if return_info:
-        out += (info,)
-    if return_qT:
-        out += (q[0],)
-    return out
+    return out + tuple(rest)
 
 
 def terminal_states(**columns):
