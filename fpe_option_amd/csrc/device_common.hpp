@@ -52,7 +52,8 @@ struct Inst {
     long long o_fs, o_fv;     // 1D factors: 5*nsb*7 and 6*nvb*7 (array FC)
     long long o_st;           // stencils Ms, Ks: 98 planes of N (array ST)
     long long o_vec;          // D, Dinv, q0, qT, bt, mw, scratch... (array VC, 12*N)
-    long long o_eig;          // Qs, Qv, ls, lv, As, Av (array EG)
+    long long o_eig;          // Qv, QvT, lv, Av (array EG)
+    long long o_eig_s;        // Qs, ls, As (array EG; shared by instances with the same s basis)
     long long o_hist;         // accepted-state history (array HS), hist_cap * N
     double kappa, theta, eta, rho, r, T, S0, V0, jac, s_lo;
     double ic_sig_s, ic_sig_v, rtol, atol;

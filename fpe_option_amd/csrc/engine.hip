@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <unordered_map>
 
 #include "assemble.inl"
 #include "engine.hpp"
@@ -340,8 +341,8 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
         o_st += 98 * N;
         I.o_vec = o_vc;
         o_vc += (long long)V_COUNT * N;
-        I.o_eig = o_eg;
-        o_eg += 2LL * I.nsb * I.nsb + 3LL * I.nvb * I.nvb + I.nsb + I.nvb;
+        I.o_eig = o_eg;  // the v part; the shared s parts follow all of them (below)
+        o_eg += 3LL * I.nvb * I.nvb + I.nvb;
         I.o_hist = o_hs;
         o_hs += (long long)opt.hist_cap * N;
         ow[s] = o_wm;
@@ -359,6 +360,49 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
         max_nm = std::max(max_nm, N);
         max_m = std::max(max_m, I.nvb);
         any_direct |= (sp.complex_mode == 1 || sp.precision == 1);
+    }
+    // s eigenproblems: one per distinct s basis and quadrature (the s mass factor,
+    // hence Qs, ls and As, depends on nothing else; European options and the
+    // vanilla solves of in-options on one grid all share it). Candidates are
+    // bucketed by a hash of a few sizes and entries, then compared exactly.
+    std::vector<int> s_reps;
+    {
+        std::unordered_map<unsigned long long, std::vector<int>> buckets;
+        auto bits = [](double x) {
+            unsigned long long u;
+            std::memcpy(&u, &x, sizeof u);
+            return u;
+        };
+        for (int s = 0; s < n; ++s) {
+            const Domain& d = doms[s];
+            unsigned long long h = 1469598103934665603ull;
+            auto mix = [&](unsigned long long v) { h = (h ^ v) * 1099511628211ull; };
+            mix(hi[s].nsb), mix(hi[s].nqs), mix(hi[s].nsk), mix(hi[s].s_lneu), mix(hi[s].s_rneu);
+            mix(bits(d.s_knots[d.s_knots.size() / 2])), mix(bits(d.s_x[d.s_x.size() / 2]));
+            mix(bits(d.s_w[d.s_w.size() / 3]));
+            // bitwise equality (equal inputs give bit-identical factors and eigenvectors)
+            auto same = [](const std::vector<double>& a, const std::vector<double>& b) {
+                return a.size() == b.size() && std::memcmp(a.data(), b.data(), sizeof(double) * a.size()) == 0;
+            };
+            std::vector<int>& b = buckets[h];
+            int rep = -1;
+            for (int c : b) {
+                const Domain& e = doms[c];
+                if (hi[c].nsb == hi[s].nsb && hi[c].s_lneu == hi[s].s_lneu && hi[c].s_rneu == hi[s].s_rneu &&
+                    same(e.s_knots, d.s_knots) && same(e.s_x, d.s_x) && same(e.s_w, d.s_w)) {
+                    rep = c;
+                    break;
+                }
+            }
+            if (rep < 0) {
+                b.push_back(s);
+                s_reps.push_back(s);
+                hi[s].o_eig_s = o_eg;
+                o_eg += 2LL * hi[s].nsb * hi[s].nsb + hi[s].nsb;
+            } else {
+                hi[s].o_eig_s = hi[rep].o_eig_s;
+            }
+        }
     }
     max_panel = (max_panel + 31) & ~31LL;  // panels start 256-byte aligned (16-byte stores in assembly)
     SlotLayout L{};
@@ -386,13 +430,16 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
     // LDS when the class budget holds both), then ADMM
     std::vector<int> ord_e0, ord_e1, ord_a;
     std::vector<IcGroup> grp_e0, grp_e1, grp_a;
-    ic_groups(n, [&](int s) { return eig_class(hi[s].nsb); },
-              [&](int s) { return double(hi[s].nsb) * hi[s].nsb * hi[s].nsb; }, ord_e0, grp_e0);
+    ic_groups(int(s_reps.size()), [&](int k) { return eig_class(hi[s_reps[k]].nsb); },
+              [&](int k) { return double(hi[s_reps[k]].nsb) * hi[s_reps[k]].nsb * hi[s_reps[k]].nsb; }, ord_e0,
+              grp_e0);
+    for (int& k : ord_e0) k = s_reps[k];  // group order over instances (one per distinct s basis)
     ic_groups(n, [&](int s) { return eig_class(hi[s].nvb); },
               [&](int s) { return double(hi[s].nvb) * hi[s].nvb * hi[s].nvb; }, ord_e1, grp_e1);
     ic_groups(n, [&](int s) { return admm_key(hi[s].nsb, hi[s].nvb); },
               [&](int s) { return double(hi[s].nsb) * hi[s].nvb * (hi[s].nsb + hi[s].nvb); }, ord_a, grp_a);
-    const long long oo_e0 = n, oo_e1 = 2LL * n, oo_a = 3LL * n;  // offsets of the group orders after the Radau order
+    // offsets of the group orders after the Radau order
+    const long long oo_e0 = n, oo_e1 = oo_e0 + (long long)ord_e0.size(), oo_a = oo_e1 + n;
     order.insert(order.end(), ord_e0.begin(), ord_e0.end());
     order.insert(order.end(), ord_e1.begin(), ord_e1.end());
     order.insert(order.end(), ord_a.begin(), ord_a.end());

@@ -14,17 +14,21 @@
 
 namespace fpe {
 
-// EG layout per instance: Qs (ns^2), Qv (nv^2), QvT (nv^2), ls (ns), lv (nv), As (ns^2), Av (nv^2)
+// EG layout: the s part Qs (ns^2), ls (ns), As (ns^2) at o_eig_s, shared by
+// every instance of the batch with the same s basis and quadrature (the s mass
+// factor, hence As = Ds Ns Ds, depends on nothing else); the v part Qv (nv^2),
+// QvT (nv^2), lv (nv), Av (nv^2) at o_eig, one per instance.
 struct EigView {
     double *Qs, *Qv, *QvT, *ls, *lv, *As, *Av;
-    __device__ EigView(double* base, int ns, int nv) {
-        Qs = base;
-        Qv = Qs + (long long)ns * ns;
-        QvT = Qv + (long long)nv * nv;
-        ls = QvT + (long long)nv * nv;
-        lv = ls + ns;
-        As = lv + nv;
-        Av = As + (long long)ns * ns;
+    __device__ EigView(double* EG, const Inst& in) {
+        const long long ns = in.nsb, nv = in.nvb;
+        Qs = EG + in.o_eig_s;
+        ls = Qs + ns * ns;
+        As = ls + ns;
+        Qv = EG + in.o_eig;
+        QvT = Qv + nv * nv;
+        lv = QvT + nv * nv;
+        Av = lv + nv;
     }
 };
 
@@ -113,7 +117,7 @@ __global__ void __launch_bounds__(ENT) k_eig(const Inst* __restrict__ inst, cons
     extern __shared__ double dyn[];
     const Inst& in = inst[order[blockIdx.x]];  // largest instances first
     const int ns = in.nsb, nv = in.nvb;
-    EigView E(EG + in.o_eig, ns, nv);
+    EigView E(EG, in);
     const int n = ax == 0 ? ns : nv;
     const double* F = FC + (ax == 0 ? in.o_fs : in.o_fv);  // factor 0 = mass
     const bool a_lds = 8LL * n * (n + 1) <= lds_bytes;
@@ -349,7 +353,7 @@ __global__ void __launch_bounds__(ANW * 64) k_admm(const Inst* __restrict__ inst
     extern __shared__ double dyn[];
     const Inst& in = inst[order[blockIdx.x]];
     const int ns = in.nsb, nv = in.nvb, N = ns * nv;
-    EigView E(const_cast<double*>(EG) + in.o_eig, ns, nv);
+    EigView E(const_cast<double*>(EG), in);
     double* base = VC + in.o_vec;
     const double* Di = base + (long long)V_DINV * N;
     const double* bt = base + (long long)V_BT * N;
