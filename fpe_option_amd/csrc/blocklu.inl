@@ -634,9 +634,13 @@ __device__ void schur_pass_gen(GAS T* __restrict__ Pr, GAS T* __restrict__ Pi, l
 // stencils, for block_lu: a coalesced zero fill of the blocks it reads before
 // writing them, then the (at most) 49 stencil entries of every row (stencil
 // planes are row-contiguous). Panels are 16-byte aligned.
-template <bool CPLX, class T>
+// PAIR: also the complex E2 = (cr + i ci) M_s + h K_s into (Qr, Qi) from the
+// same stencil loads (the real E into Pr, with ar; ai unused).
+template <bool CPLX, class T, bool PAIR = false>
 __device__ void assemble_E(const double* __restrict__ Ms, const double* __restrict__ Ks, int ns, int m,
-                           double ar, double ai, double h, T* __restrict__ Pr, T* __restrict__ Pi) {
+                           double ar, double ai, double h, T* __restrict__ Pr, T* __restrict__ Pi,
+                           double cr = 0.0, double ci = 0.0, T* __restrict__ Qr = nullptr, T* __restrict__ Qi = nullptr) {
+    static_assert(!(PAIR && CPLX), "the pair form writes one real and one complex matrix");
     const int N = ns * m, ld = 7 * m;
     // Zeros go only where block_lu reads entries it has not written: block rows
     // 0..3 whole (row 0 and column 0 are never updated), and the outermost
@@ -649,23 +653,29 @@ __device__ void assemble_E(const double* __restrict__ Ms, const double* __restri
     for (int e = threadIdx.x; e < head / V; e += RNT) {
         reinterpret_cast<double2*>(Pr)[e] = z;
         if (CPLX) reinterpret_cast<double2*>(Pi)[e] = z;
+        if (PAIR) {
+            reinterpret_cast<double2*>(Qr)[e] = z;
+            reinterpret_cast<double2*>(Qi)[e] = z;
+        }
     }
     for (int e = head / V * V + threadIdx.x; e < head; e += RNT) {
         Pr[e] = T(0);
         if (CPLX) Pi[e] = T(0);
+        if (PAIR) Qr[e] = Qi[e] = T(0);
     }
     {
         const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
         for (int row = 4 * m + w; row < N; row += RNT / 64) {
             const int I = row / m;
-            T* pr = Pr + (long long)row * ld;
-            T* pi = CPLX ? Pi + (long long)row * ld : nullptr;
+            const long long o = (long long)row * ld;
             if (lane < m) {
-                pr[lane] = T(0);  // d = 0 (block row I >= 4 > 3)
-                if (CPLX) pi[lane] = T(0);
+                Pr[o + lane] = T(0);  // d = 0 (block row I >= 4 > 3)
+                if (CPLX) Pi[o + lane] = T(0);
+                if (PAIR) Qr[o + lane] = Qi[o + lane] = T(0);
                 if (I + 3 < ns) {
-                    pr[6 * m + lane] = T(0);
-                    if (CPLX) pi[6 * m + lane] = T(0);
+                    Pr[o + 6 * m + lane] = T(0);
+                    if (CPLX) Pi[o + 6 * m + lane] = T(0);
+                    if (PAIR) Qr[o + 6 * m + lane] = Qi[o + 6 * m + lane] = T(0);
                 }
             }
         }
@@ -700,14 +710,17 @@ __device__ void assemble_E(const double* __restrict__ Ms, const double* __restri
             for (int g = 0; g < 2; ++g) {
                 const int I = rc[g] / m, i = rc[g] - I * m, j = i + q - 3;
                 if (act[g] && j >= 0 && j < m) {
-                    T* pr = Pr + (long long)rc[g] * ld + j;
-                    T* pi = CPLX ? Pi + (long long)rc[g] * ld + j : nullptr;
+                    const long long o = (long long)rc[g] * ld + j;
 #pragma unroll
                     for (int d = 0; d < 7; ++d) {
                         const int J = I + d - 3;
                         if (J >= 0 && J < ns) {
-                            pr[d * m] = T(ar * mv[g][d] + h * kv[g][d]);
-                            if (CPLX) pi[d * m] = T(ai * mv[g][d]);
+                            Pr[o + d * m] = T(ar * mv[g][d] + h * kv[g][d]);
+                            if (CPLX) Pi[o + d * m] = T(ai * mv[g][d]);
+                            if (PAIR) {
+                                Qr[o + d * m] = T(cr * mv[g][d] + h * kv[g][d]);
+                                Qi[o + d * m] = T(ci * mv[g][d]);
+                            }
                         }
                     }
                 }

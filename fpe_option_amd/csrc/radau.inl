@@ -363,16 +363,22 @@ __global__ void __launch_bounds__(RNT, RADAU_WG_PER_CU * RNW / 4) k_radau(const 
             if (need_lu) {
                 tk_mark = wall_clock64();
                 int bad = 0;
+                // the real and (direct mode) complex stage matrices from one pass over the stencils
                 if (mixed) {
-                    assemble_E<false, float>(Ms, Ks, ns, m, U1, 0.0, h, Pf, (float*)nullptr);
+                    assemble_E<false, float, true>(Ms, Ks, ns, m, U1, 0.0, h, Pf, (float*)nullptr, ALPH, BETA, Crf,
+                                                   Cif);
                     __syncthreads();
                     lap(tk_asm);
                     bad = block_lu<false, float>(Pf, (float*)nullptr, ns, m, in.lu_var, S.lu);
+                    bad |= block_lu<true, float>(Crf, Cif, ns, m, in.lu_var, S.lu);
                     lap(tk_lu);
-                    assemble_E<true, float>(Ms, Ks, ns, m, ALPH, BETA, h, Crf, Cif);
+                } else if (direct) {
+                    assemble_E<false, double, true>(Ms, Ks, ns, m, U1, 0.0, h, P, (double*)nullptr, ALPH, BETA, Cr,
+                                                    Ci);
                     __syncthreads();
                     lap(tk_asm);
-                    bad |= block_lu<true, float>(Crf, Cif, ns, m, in.lu_var, S.lu);
+                    bad = block_lu<false, double>(P, (double*)nullptr, ns, m, in.lu_var, S.lu);
+                    bad |= block_lu<true, double>(Cr, Ci, ns, m, in.lu_var, S.lu);
                     lap(tk_lu);
                 } else {
                     assemble_E<false, double>(Ms, Ks, ns, m, U1, 0.0, h, P, (double*)nullptr);
@@ -380,13 +386,6 @@ __global__ void __launch_bounds__(RNT, RADAU_WG_PER_CU * RNW / 4) k_radau(const 
                     lap(tk_asm);
                     bad = block_lu<false, double>(P, (double*)nullptr, ns, m, in.lu_var, S.lu);
                     lap(tk_lu);
-                    if (direct) {
-                        assemble_E<true, double>(Ms, Ks, ns, m, ALPH, BETA, h, Cr, Ci);
-                        __syncthreads();
-                        lap(tk_asm);
-                        bad |= block_lu<true, double>(Cr, Ci, ns, m, in.lu_var, S.lu);
-                        lap(tk_lu);
-                    }
                 }
                 h_lu = h;
                 ++n_lu;
