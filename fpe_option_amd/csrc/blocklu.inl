@@ -1580,6 +1580,166 @@ __device__ __forceinline__ void block_solve1(const T* Pr, const T* Pi, int ns, i
 }
 
 // Block solve with NR right-hand sides, compiled for the common block widths.
+// The real solve of x with the real factors P and the complex solve of (zr, zi)
+// with the complex factors (Cr, Ci) in one sweep per direction (the two stage
+// systems of a direct-mode Newton iteration): each block-row step issues both
+// solves' factor loads before either contracts, so the two recurrences share one
+// memory latency and one barrier per block row. The per-entry arithmetic is that
+// of block_solve1_w for each solve, so the results are the same. NF > 0 only
+// (m / 4 known; the factor rows of both solves in registers); ring: LDS of at
+// least 15 * 64 doubles.
+template <int NF>
+__device__ __noinline__ void block_solve_pair_w(const double* __restrict__ P_, const double* __restrict__ Cr_,
+                                                const double* __restrict__ Ci_, int ns, int m,
+                                                double* __restrict__ x_, double* __restrict__ zr_,
+                                                double* __restrict__ zi_, double* ring) {
+    static_assert(NF > 0, "the paired solve is compiled for known m / 4");
+    GAS const double* const P = uniform(gas(P_));
+    GAS const double* const Cr = uniform(gas(Cr_));
+    GAS const double* const Ci = uniform(gas(Ci_));
+    GAS double* const x = uniform(gas(x_));
+    GAS double* const zr = uniform(gas(zr_));
+    GAS double* const zi = uniform(gas(zi_));
+    ns = uniform(ns), m = uniform(m);
+    const int ld = 7 * m;
+    const long long ps = (long long)m * ld;
+    constexpr int TPR = RNT / MAX_BLOCK;  // threads per row
+    const int tid = threadIdx.x, i = tid / TPR, q = tid % TPR;
+    const bool row = i < m;
+    double* rx = ring;             // real solution blocks (4 x 64)
+    double* rzr = ring + 4 * 64;   // complex, re
+    double* rzi = ring + 8 * 64;   // complex, im
+    double* tx = ring + 12 * 64;
+    double* tzr = ring + 13 * 64;
+    double* tzi = ring + 14 * 64;
+    constexpr int UA = NF + 1;   // column groups held per block
+    const int ic = min(i, m - 1);  // rows past m read row m - 1, unstored
+    double a0[UA], a1[UA], c0[UA], e0[UA], c1[UA], e1[UA];  // real rows a; complex rows (c, e) = (re, im)
+    auto fill = [&](GAS const double* qr, bool ok, double* a) {
+        if (ok) {
+#pragma unroll
+            for (int u = 0; u < NF; ++u) a[u] = qr[q + TPR * u];
+            const int k = q + TPR * NF;
+            a[NF] = k < m ? qr[k] : 0.0;
+        } else {
+#pragma unroll
+            for (int u = 0; u < UA; ++u) a[u] = 0.0;
+        }
+    };
+    // factor entries (row i, columns col + q + 4u) of block row I, both solves
+    auto load_row = [&](int I, int col, bool ok, double* a, double* c, double* e) {
+        const long long o = I * ps + (long long)ic * ld + col;
+        fill(P + o, ok, a);
+        fill(Cr + o, ok, c);
+        fill(Ci + o, ok, e);
+    };
+    auto contract_r = [&](const double* a, const double* y, double& ar) {
+#pragma unroll
+        for (int u = 0; u < UA; ++u) {
+            const int k = q + TPR * u;
+            if (u < NF || k < m) ar += a[u] * y[k];
+        }
+    };
+    auto contract_c = [&](const double* a, const double* b, const double* yr, const double* yi, double& ar,
+                          double& ai) {
+#pragma unroll
+        for (int u = 0; u < UA; ++u) {
+            const int k = q + TPR * u;
+            if (u < NF || k < m) {
+                ar += a[u] * yr[k] - b[u] * yi[k];
+                ai += a[u] * yi[k] + b[u] * yr[k];
+            }
+        }
+    };
+    auto reduce = [&](double& v) {
+#pragma unroll
+        for (int o = 1; o < TPR; o <<= 1) v += __shfl_xor(v, o, 64);
+    };
+    auto copy = [&](double* dst, const double* src) {
+#pragma unroll
+        for (int u = 0; u < UA; ++u) dst[u] = src[u];
+    };
+    // ring slots of blocks left of the matrix are contracted with zero factor
+    // entries and never written: start from zeros
+    __syncthreads();
+    for (int e = tid; e < 12 * 64; e += RNT) ring[e] = 0.0;
+    __syncthreads();
+    // forward: y_I = b_I - sum_{d=0..2} L_{I, I-3+d} y_{I-3+d} (ring slot of block J: J & 3)
+    auto d0_of = [&](int I) { return I >= 3 ? 0 : 3 - I; };
+    if (ns > 0) load_row(0, 0, d0_of(0) <= 0, a0, c0, e0);
+    for (int I = 0; I < ns; ++I) {
+        const int d0 = d0_of(I);
+        const int s1 = ((I + 1) & 3) * 64, s2 = ((I + 2) & 3) * 64, s3 = ((I + 3) & 3) * 64;
+        double ax = 0.0, azr = 0.0, azi = 0.0;
+        load_row(I, m, d0 <= 1, a1, c1, e1);
+        contract_r(a0, rx + s1, ax);
+        contract_c(c0, e0, rzr + s1, rzi + s1, azr, azi);
+        load_row(I, 2 * m, d0 <= 2, a0, c0, e0);
+        contract_r(a1, rx + s2, ax);
+        contract_c(c1, e1, rzr + s2, rzi + s2, azr, azi);
+        if (I + 1 < ns) load_row(I + 1, 0, d0_of(I + 1) <= 0, a1, c1, e1);
+        contract_r(a0, rx + s3, ax);
+        contract_c(c0, e0, rzr + s3, rzi + s3, azr, azi);
+        copy(a0, a1), copy(c0, c1), copy(e0, e1);
+        reduce(ax), reduce(azr), reduce(azi);
+        if (row && q == 0) {
+            const int sl = (I & 3) * 64 + i;
+            const double vx = x[I * m + i] - ax, vr = zr[I * m + i] - azr, vi = zi[I * m + i] - azi;
+            rx[sl] = vx, rzr[sl] = vr, rzi[sl] = vi;
+            x[I * m + i] = vx, zr[I * m + i] = vr, zi[I * m + i] = vi;
+        }
+        __syncthreads();
+    }
+    // backward: x_I = Ainv_I (y_I - sum_{d=0..2} U_{I, I+1+d} x_{I+1+d})
+    auto nK_of = [&](int I) { return (ns - 1 - I) < 3 ? (ns - 1 - I) : 3; };
+    if (ns > 0) load_row(ns - 1, 4 * m, nK_of(ns - 1) > 0, a0, c0, e0);
+    for (int I = ns - 1; I >= 0; --I) {
+        const int nK = nK_of(I);
+        const int s1 = ((I + 1) & 3) * 64, s2 = ((I + 2) & 3) * 64, s3 = ((I + 3) & 3) * 64;
+        double ax = 0.0, azr = 0.0, azi = 0.0;
+        load_row(I, 5 * m, nK > 1, a1, c1, e1);
+        contract_r(a0, rx + s1, ax);
+        contract_c(c0, e0, rzr + s1, rzi + s1, azr, azi);
+        load_row(I, 6 * m, nK > 2, a0, c0, e0);
+        contract_r(a1, rx + s2, ax);
+        contract_c(c1, e1, rzr + s2, rzi + s2, azr, azi);
+        load_row(I, 3 * m, true, a1, c1, e1);  // Ainv_I rows, used after the barrier
+        contract_r(a0, rx + s3, ax);
+        contract_c(c0, e0, rzr + s3, rzi + s3, azr, azi);
+        if (I > 0) load_row(I - 1, 4 * m, nK_of(I - 1) > 0, a0, c0, e0);
+        reduce(ax), reduce(azr), reduce(azi);
+        if (row && q == 0) {
+            tx[i] = x[I * m + i] - ax;
+            tzr[i] = zr[I * m + i] - azr;
+            tzi[i] = zi[I * m + i] - azi;
+        }
+        __syncthreads();
+        ax = 0.0, azr = 0.0, azi = 0.0;
+        contract_r(a1, tx, ax);
+        contract_c(c1, e1, tzr, tzi, azr, azi);
+        reduce(ax), reduce(azr), reduce(azi);
+        if (row && q == 0) {
+            const int sl = (I & 3) * 64 + i;
+            x[I * m + i] = ax, rx[sl] = ax;
+            zr[I * m + i] = azr, rzr[sl] = azr;
+            zi[I * m + i] = azi, rzi[sl] = azi;
+        }
+        __syncthreads();
+    }
+}
+
+// The paired real and complex solve where it is compiled (m / 4 in {5, 7, 10});
+// returns false for other block widths, which take the two single solves.
+__device__ __forceinline__ bool block_solve_pair(const double* P, const double* Cr, const double* Ci, int ns, int m,
+                                                 double* x, double* zr, double* zi, double* ring) {
+    switch (m >> 2) {
+        case 5: block_solve_pair_w<5>(P, Cr, Ci, ns, m, x, zr, zi, ring); return true;
+        case 7: block_solve_pair_w<7>(P, Cr, Ci, ns, m, x, zr, zi, ring); return true;
+        case 10: block_solve_pair_w<10>(P, Cr, Ci, ns, m, x, zr, zi, ring); return true;
+        default: return false;
+    }
+}
+
 template <int NR>
 __device__ __forceinline__ void block_solve_multi(const double* P, int ns, int m, double* x0, double* x1, double* x2,
                                                   double* ring) {

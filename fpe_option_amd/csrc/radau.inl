@@ -442,8 +442,11 @@ __global__ void __launch_bounds__(RNT, RADAU_WG_PER_CU * RNW / 4) k_radau(const 
                 __syncthreads();
                 lap(tk_mv);
                 if (direct) {
-                    solve_real(vv(S_DF1));
-                    solve_cplx(vv(S_DF2), vv(S_DF3));
+                    // FP64 factors of a compiled block width: both stage systems in one sweep
+                    if (mixed || !block_solve_pair(P, Cr, Ci, ns, m, vv(S_DF1), vv(S_DF2), vv(S_DF3), ring)) {
+                        solve_real(vv(S_DF1));
+                        solve_cplx(vv(S_DF2), vv(S_DF3));
+                    }
                     lap(tk_solve);
                 } else {
                     // dF1 and the preconditioned GMRES right-hand side in one pass
