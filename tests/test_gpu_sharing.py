@@ -1,0 +1,44 @@
+"""Work shared inside a batch does not change any result. Options on the same
+s grid (European options and the vanilla legs of in-options) share one s-axis
+eigenproblem of the initial-condition solve, and the initial-condition kernels
+run in launch groups by grid shape; each option of a mixed batch must price
+bit for bit as it does alone in a batch of one."""
+import numpy as np
+import pytest
+
+from fpe_option_amd import api, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def _single(cols, i):
+    return api.price_batch(**workloads.take(cols, np.array([i])))
+
+
+def test_batch_prices_equal_single_option_prices():
+    cols = workloads.batch_columns(24, seed=11)
+    # three grids, so that several options share each s grid and the launch groups mix shapes
+    cols["n_s"] = np.array([40, 60, 40] * 8)
+    cols["n_v"] = np.array([20, 30, 20] * 8)
+    prices, status = api.price_batch(**cols)
+    assert np.all(status == 0)
+    for i in range(24):
+        p1, s1 = _single(cols, i)
+        assert s1[0] == 0
+        assert p1[0] == prices[i], (i, p1[0], prices[i])
+
+
+def test_shared_s_grid_across_types_and_strikes():
+    # one grid: every European and in-option vanilla leg shares the s eigenproblem
+    n = 12
+    cols = dict(kappa=np.linspace(0.8, 3.0, n), theta=0.05, sigma=0.4, rho=-0.5, r=0.03, T=0.5, S0=60.0,
+                V0=np.linspace(0.04, 0.2, n), strike=np.linspace(50.0, 70.0, n),
+                option_type=np.array([8, 9, 6, 7, 8, 9, 4, 5, 8, 9, 0, 1]),
+                barrier=np.array([0, 0, 72, 72, 0, 0, 70, 70, 0, 0, 48, 48], dtype=float),
+                n_s=40, n_v=20, disc_mode=1, v_max=1.5, s_min=0.0, s_max=180.0, ic_sigma_s=0.5, ic_sigma_v=0.005,
+                complex_mode=1)
+    prices, status = api.price_batch(**cols)
+    assert np.all(status == 0)
+    for i in range(n):
+        p1, s1 = _single(cols, i)
+        assert s1[0] == 0 and p1[0] == prices[i], (i, p1[0], prices[i])
