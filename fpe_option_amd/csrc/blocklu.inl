@@ -1587,16 +1587,17 @@ __device__ __forceinline__ void block_solve1(const T* Pr, const T* Pi, int ns, i
 // memory latency and one barrier per block row. The per-entry arithmetic is that
 // of block_solve1_w for each solve, so the results are the same. NF > 0 only
 // (m / 4 known; the factor rows of both solves in registers); ring: LDS of at
-// least 15 * 64 doubles.
-template <int NF>
-__device__ __noinline__ void block_solve_pair_w(const double* __restrict__ P_, const double* __restrict__ Cr_,
-                                                const double* __restrict__ Ci_, int ns, int m,
+// least 15 * 64 doubles. Factors of type T (FP64, or FP32 for the mixed-precision
+// path) are widened to FP64; all arithmetic is FP64.
+template <class T, int NF>
+__device__ __noinline__ void block_solve_pair_w(const T* __restrict__ P_, const T* __restrict__ Cr_,
+                                                const T* __restrict__ Ci_, int ns, int m,
                                                 double* __restrict__ x_, double* __restrict__ zr_,
                                                 double* __restrict__ zi_, double* ring) {
     static_assert(NF > 0, "the paired solve is compiled for known m / 4");
-    GAS const double* const P = uniform(gas(P_));
-    GAS const double* const Cr = uniform(gas(Cr_));
-    GAS const double* const Ci = uniform(gas(Ci_));
+    GAS const T* const P = uniform(gas(P_));
+    GAS const T* const Cr = uniform(gas(Cr_));
+    GAS const T* const Ci = uniform(gas(Ci_));
     GAS double* const x = uniform(gas(x_));
     GAS double* const zr = uniform(gas(zr_));
     GAS double* const zi = uniform(gas(zi_));
@@ -1615,12 +1616,12 @@ __device__ __noinline__ void block_solve_pair_w(const double* __restrict__ P_, c
     constexpr int UA = NF + 1;   // column groups held per block
     const int ic = min(i, m - 1);  // rows past m read row m - 1, unstored
     double a0[UA], a1[UA], c0[UA], e0[UA], c1[UA], e1[UA];  // real rows a; complex rows (c, e) = (re, im)
-    auto fill = [&](GAS const double* qr, bool ok, double* a) {
+    auto fill = [&](GAS const T* qr, bool ok, double* a) {
         if (ok) {
 #pragma unroll
-            for (int u = 0; u < NF; ++u) a[u] = qr[q + TPR * u];
+            for (int u = 0; u < NF; ++u) a[u] = double(qr[q + TPR * u]);
             const int k = q + TPR * NF;
-            a[NF] = k < m ? qr[k] : 0.0;
+            a[NF] = k < m ? double(qr[k]) : 0.0;
         } else {
 #pragma unroll
             for (int u = 0; u < UA; ++u) a[u] = 0.0;
@@ -1730,12 +1731,13 @@ __device__ __noinline__ void block_solve_pair_w(const double* __restrict__ P_, c
 
 // The paired real and complex solve where it is compiled (m / 4 in {5, 7, 10});
 // returns false for other block widths, which take the two single solves.
-__device__ __forceinline__ bool block_solve_pair(const double* P, const double* Cr, const double* Ci, int ns, int m,
-                                                 double* x, double* zr, double* zi, double* ring) {
+template <class T>
+__device__ __forceinline__ bool block_solve_pair(const T* P, const T* Cr, const T* Ci, int ns, int m, double* x,
+                                                 double* zr, double* zi, double* ring) {
     switch (m >> 2) {
-        case 5: block_solve_pair_w<5>(P, Cr, Ci, ns, m, x, zr, zi, ring); return true;
-        case 7: block_solve_pair_w<7>(P, Cr, Ci, ns, m, x, zr, zi, ring); return true;
-        case 10: block_solve_pair_w<10>(P, Cr, Ci, ns, m, x, zr, zi, ring); return true;
+        case 5: block_solve_pair_w<T, 5>(P, Cr, Ci, ns, m, x, zr, zi, ring); return true;
+        case 7: block_solve_pair_w<T, 7>(P, Cr, Ci, ns, m, x, zr, zi, ring); return true;
+        case 10: block_solve_pair_w<T, 10>(P, Cr, Ci, ns, m, x, zr, zi, ring); return true;
         default: return false;
     }
 }

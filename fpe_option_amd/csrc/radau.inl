@@ -47,7 +47,10 @@ enum {
     S_Y = 0, S_W, S_SCAL, S_Z1, S_Z2, S_Z3, S_F1, S_F2, S_F3, S_KZ1, S_KZ2, S_KZ3, S_MF1, S_MF2, S_MF3,
     S_RR, S_RRE, S_RIM, S_DF1, S_DF2, S_DF3, S_C0, S_C1, S_C2, S_C3, S_CERR, S_MC, S_ERR, S_TMP, S_WRE, S_WIM,
     S_RB, S_RBI, S_RT, S_RTI,  // iterative refinement (mixed precision): right-hand side, residual/correction
-    S_VRE0, S_VIM0 = S_VRE0 + GM_K + 1, S_NVEC = S_VIM0 + GM_K + 1
+    S_VRE0, S_VIM0 = S_VRE0 + GM_K + 1, S_NVEC = S_VIM0 + GM_K + 1,
+    // the real system's refinement vectors while both stage systems refine together:
+    // mixed precision never runs GMRES, so they take two of its basis slots
+    S_RB1 = S_VRE0, S_RT1 = S_VRE0 + 1
 };
 
 struct SlotLayout {
@@ -207,6 +210,54 @@ __device__ void stage_residual(const double* __restrict__ Ms, const double* __re
     }
 }
 
+// The residuals of both stage systems in one pass over the stencils:
+// t1 = b1 - (a1 M + h K) x1 (when R1) and (t2r, t2i) = (b2r, b2i) -
+// ((a2 + i beta) M + h K)(x2r, x2i) (when R2), each with stage_residual's
+// per-entry arithmetic; a row's stencil window is loaded once for both.
+template <bool R1, bool R2>
+__device__ void stage_residual_pair(const double* __restrict__ Ms, const double* __restrict__ Ks, int ns, int nv,
+                                    double a1, double a2, double beta, double h, const double* x1, const double* b1,
+                                    double* t1, const double* x2r, const double* x2i, const double* b2r,
+                                    const double* b2i, double* t2r, double* t2i) {
+    const int N = ns * nv;
+    for (int r = threadIdx.x; r < N; r += RNT) {
+        double e1 = 0, er = 0, ei = 0, mr = 0, mi = 0;
+#pragma unroll 1
+        for (int dJ = -3; dJ <= 3; ++dJ) {
+            int c[7];
+            double mv[7], kv[7], y1[7], yr[7], yi[7];
+            window_cols(c, r, dJ, nv, N);
+#pragma unroll
+            for (int q = 0; q < 7; ++q) {
+                const long long e = plane_at(dJ, q, N, r);
+                mv[q] = Ms[e], kv[q] = Ks[e];
+                if (R1) y1[q] = x1[c[q]];
+                if (R2) yr[q] = x2r[c[q]], yi[q] = x2i[c[q]];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int q = 0; q < 7; ++q) {
+                if (R1) {
+                    const double ed = a1 * mv[q] + h * kv[q];
+                    e1 += ed * y1[q];
+                }
+                if (R2) {
+                    const double ed = a2 * mv[q] + h * kv[q];
+                    er += ed * yr[q];
+                    ei += ed * yi[q];
+                    mr += mv[q] * yr[q];
+                    mi += mv[q] * yi[q];
+                }
+            }
+        }
+        if (R1) t1[r] = b1[r] - e1;
+        if (R2) {
+            t2r[r] = b2r[r] - (er - beta * mi);
+            t2i[r] = b2i[r] - (ei + beta * mr);
+        }
+    }
+}
+
 __global__ void __launch_bounds__(RNT, RADAU_WG_PER_CU * RNW / 4) k_radau(const Inst* __restrict__ inst, int n_inst, int* __restrict__ queue, const int* __restrict__ order,
                                               const double* __restrict__ ST, double* __restrict__ VC,
                                               double* __restrict__ WS, int* __restrict__ WI, SlotLayout L,
@@ -350,6 +401,71 @@ __global__ void __launch_bounds__(RNT, RADAU_WG_PER_CU * RNW / 4) k_radau(const 
                 __syncthreads();
             }
         };
+        // Mixed precision: the two stage systems of a Newton iteration refined in
+        // lockstep, each with exactly the iterates of solve_real / solve_cplx (a
+        // system that has met its tolerance stops changing): one stencil pass gives
+        // both residuals, and the FP32 solves of both run as one paired sweep where
+        // the block width allows.
+        auto solve_pair_mixed = [&](double* x, double* zr, double* zi) {
+            double* b1 = vv(S_RB1);
+            double* t1 = vv(S_RT1);
+            double* b2r = vv(S_RB);
+            double* b2i = vv(S_RBI);
+            double* t2r = vv(S_RT);
+            double* t2i = vv(S_RTI);
+            double bn1 = 0.0, bn2 = 0.0;
+            for (int r = tid; r < N; r += RNT) {
+                b1[r] = x[r], b2r[r] = zr[r], b2i[r] = zi[r];
+                bn1 = fmax(bn1, fabs(x[r]));
+                bn2 = fmax(bn2, fmax(fabs(zr[r]), fabs(zi[r])));
+            }
+            bn1 = block_max_w<RNW>(bn1, S.red);
+            bn2 = block_max_w<RNW>(bn2, S.red);
+            auto solve32 = [&](bool c1, bool c2, double* v1, double* vr, double* vi) {
+                if (c1 && c2 && block_solve_pair<float>(Pf, Crf, Cif, ns, m, v1, vr, vi, ring)) return;
+                if (c1) block_solve1<false, float>(Pf, nullptr, ns, m, v1, nullptr, ring);
+                if (c2) block_solve1<true, float>(Crf, Cif, ns, m, vr, vi, ring);
+            };
+            solve32(true, true, x, zr, zi);
+            bool act1 = true, act2 = true;
+            for (int it = 0;; ++it) {
+                if (act1 && act2)
+                    stage_residual_pair<true, true>(Ms, Ks, ns, nv, U1, ALPH, BETA, h_lu, x, b1, t1, zr, zi, b2r, b2i,
+                                                    t2r, t2i);
+                else if (act1)
+                    stage_residual_pair<true, false>(Ms, Ks, ns, nv, U1, ALPH, BETA, h_lu, x, b1, t1, zr, zi, b2r,
+                                                     b2i, t2r, t2i);
+                else
+                    stage_residual_pair<false, true>(Ms, Ks, ns, nv, U1, ALPH, BETA, h_lu, x, b1, t1, zr, zi, b2r,
+                                                     b2i, t2r, t2i);
+                double rn1 = 0.0, rn2 = 0.0;
+                for (int r = tid; r < N; r += RNT) {
+                    if (act1) rn1 = fmax(rn1, fabs(t1[r]));
+                    if (act2) rn2 = fmax(rn2, fmax(fabs(t2r[r]), fabs(t2i[r])));
+                }
+                rn1 = block_max_w<RNW>(rn1, S.red);
+                rn2 = block_max_w<RNW>(rn2, S.red);
+                // per system, the decisions of solve_real / solve_cplx at iteration it
+                bool c1 = false, c2 = false;
+                if (act1) {
+                    if (!(rn1 > REFINE_TOL * bn1)) act1 = false;
+                    else if (it == REFINE_MAX) refine_fail = true, act1 = false;
+                    else ++n_ref, c1 = true;
+                }
+                if (act2) {
+                    if (!(rn2 > REFINE_TOL * bn2)) act2 = false;
+                    else if (it == REFINE_MAX) refine_fail = true, act2 = false;
+                    else ++n_ref, c2 = true;
+                }
+                if (!c1 && !c2) break;
+                solve32(c1, c2, t1, t2r, t2i);
+                for (int r = tid; r < N; r += RNT) {
+                    if (c1) x[r] += t1[r];
+                    if (c2) zr[r] += t2r[r], zi[r] += t2i[r];
+                }
+                __syncthreads();
+            }
+        };
 
         while (posneg * (t1 - t) > uround * fmax(fabs(t), fabs(t1))) {
             if (n_steps > in.max_steps || ++guard > 20LL * in.max_steps) { status = ST_MAX_STEPS; break; }
@@ -442,8 +558,11 @@ __global__ void __launch_bounds__(RNT, RADAU_WG_PER_CU * RNW / 4) k_radau(const 
                 __syncthreads();
                 lap(tk_mv);
                 if (direct) {
-                    // FP64 factors of a compiled block width: both stage systems in one sweep
-                    if (mixed || !block_solve_pair(P, Cr, Ci, ns, m, vv(S_DF1), vv(S_DF2), vv(S_DF3), ring)) {
+                    // both stage systems together: FP64 factors in one sweep (compiled block
+                    // widths), mixed precision in one refinement loop
+                    if (mixed) {
+                        solve_pair_mixed(vv(S_DF1), vv(S_DF2), vv(S_DF3));
+                    } else if (!block_solve_pair<double>(P, Cr, Ci, ns, m, vv(S_DF1), vv(S_DF2), vv(S_DF3), ring)) {
                         solve_real(vv(S_DF1));
                         solve_cplx(vv(S_DF2), vv(S_DF3));
                     }
