@@ -91,6 +91,16 @@ def profiled_traffic(key):
     return float(vals["traffic_bytes_per_launch"])
 
 
+def tuning_table():
+    """The per-grid choices the library ran with (fpe_option_amd/tuning_gfx950.txt):
+    factorisation code path and initial-condition ADMM launch form per (n_sb, n_vb)."""
+    from fpe_option_amd import autotune
+    forms = {-1: "auto", 0: "waves8_full", 1: "waves4_half", 2: "waves4_third"}
+    rows = autotune.read_table(autotune.TABLE)
+    return [[nsb, nvb, "compiled_k" if v == 1 else "generic", forms.get(f, "auto")]
+            for (nsb, nvb), (v, f) in sorted(rows.items())]
+
+
 def surrogate_leg(n=1_000_000, audit_frac=0.01, n_out=256, seed=20261205, tol=1e-4, n_rate=4096):
     """The learned path on n seeded European options from the training box
     (plus n_out rows outside it): the ensemble prices every in-box option with
@@ -280,6 +290,8 @@ def main():
                      "alg_flops_per_launch": st["alg_flops"], "hbm_achieved_GBs": round(achieved_gbs, 2),
                      "hbm_frac": round(achieved_gbs / HBM_PEAK_GBS, 5)},
         "cpu_baseline": None,
+        "tuning": {"source": "fpe_option_amd/tuning_gfx950.txt",
+                   "columns": ["n_sb", "n_vb", "factorisation", "admm_launch"], "shapes": tuning_table()},
     }
     if not args.no_mixed:
         line["mixed"] = mixed_leg(api, cols, prices, status, value, args.n, 1,

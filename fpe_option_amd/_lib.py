@@ -60,7 +60,7 @@ class FpeBatchOption(C.Structure):
                                  "complex_mode", "precision", "exact_constants")] + [
         (n, C.c_double) for n in ("ic_sigma_s", "ic_sigma_v", "rtol", "atol", "knot_width_s", "knot_width_v",
                                   "knot_center_v")] + [
-        (n, C.c_int32) for n in ("lu_variant", "max_steps", "refine_max", "reserved")]
+        (n, C.c_int32) for n in ("lu_variant", "max_steps", "refine_max", "ic_variant")]
 
 
 class FpeBatchStats(C.Structure):

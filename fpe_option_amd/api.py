@@ -329,7 +329,7 @@ class Compute:
 _BATCH_FIELDS = ("kappa", "theta", "sigma", "rho", "r", "T", "S0", "V0", "s_min", "s_max", "v_max", "barrier",
                  "strike", "option_type", "n_s", "n_v", "disc_mode", "num_insert", "complex_mode", "precision",
                  "exact_constants", "ic_sigma_s", "ic_sigma_v", "rtol", "atol", "knot_width_s", "knot_width_v",
-                 "knot_center_v", "lu_variant", "max_steps", "refine_max")
+                 "knot_center_v", "lu_variant", "max_steps", "refine_max", "ic_variant")
 
 
 def _batch_array(columns):

@@ -1,16 +1,25 @@
-"""Per-grid autotuner of the block-banded factorisation (run on the GPU).
+"""Per-grid autotuner of the block-banded factorisation and of the
+initial-condition launch form (run on the GPU).
 
 For every grid shape (n_s, n_v) in {40, 60, 80, 120} x {20, 30, 40, 60} it
 prices the same seeded single-shape batch (the 4096-option parameter box,
-span discretisation, direct complex stage solve) once with each factorisation
-code path -- the generic loops (one code path for every block width) and the
-MFMA loops compiled for the block width's k-step count -- and keeps the one
-with the shorter Radau kernel time. The choice is written to
-tuning_gfx950.txt next to libfpe_engine.so (read by the library at first use,
-one line "n_sb n_vb variant" per shape, variant 0 generic / 1 compiled) and the
-measurements to tuning_gfx950.json.
+span discretisation, direct complex stage solve)
+  * once with each factorisation code path -- the generic loops (one code path
+    for every block width) and the MFMA loops compiled for the block width's
+    k-step count -- and keeps the one with the shorter Radau kernel time;
+  * once with each ADMM launch form of the initial condition (FpeBatchOption
+    ic_variant 1..3: 8-wave workgroups with the CU's LDS, 4-wave ones in a half
+    or a third of it) and once with the library's own choice (ic_variant 0, the
+    most resident workgroups per CU), on a short maturity so the Radau phase
+    stays small, and keeps a form only where it beats that choice by more than
+    1 % of the initial-condition time.
+The choices are written to tuning_gfx950.txt next to libfpe_engine.so (read by
+the library at first use, one line "n_sb n_vb variant admm_form" per shape:
+variant 0 generic / 1 compiled; admm_form 0..2, or -1 for the library's choice)
+and the measurements to tuning_gfx950.json. --ic-only keeps the factorisation
+column of the existing table and measures the launch forms alone.
 
-Usage: python -m fpe_option_amd.autotune [--n OPTIONS] [--reps R] [--out DIR]
+Usage: python -m fpe_option_amd.autotune [--n OPTIONS] [--reps R] [--ic-only] [--out DIR]
 """
 import argparse
 import json
@@ -48,31 +57,87 @@ def measure(shape, n, reps):
     return dims, out, d
 
 
+IC_FORMS = {0: "auto", 1: "waves8_full", 2: "waves4_half", 3: "waves4_third"}  # FpeBatchOption.ic_variant
+
+
+def measure_ic(shape, n, reps):
+    """Initial-condition time (ms) of one single-shape batch under each launch form."""
+    cols = workloads.batch_columns(n, grid=shape)
+    cols["T"] = np.full(n, 0.02)
+    out, ref = {}, None
+    for v in IC_FORMS:
+        c = dict(cols, ic_variant=v)
+        api.price_batch(**c)  # warm-up
+        ic = []
+        for _ in range(reps):
+            prices, status, st = api.price_batch(return_stats=True, **c)
+            assert np.all(status == 0), f"{shape} ic form {IC_FORMS[v]}: status {np.unique(status)}"
+            ic.append(st["ms_ic"])
+        out[IC_FORMS[v]] = float(np.median(ic))
+        # the forms differ only in how the residual norms are summed
+        ref = prices if ref is None else ref
+        out.setdefault("max_abs_dprice_between_forms", 0.0)
+        out["max_abs_dprice_between_forms"] = max(out["max_abs_dprice_between_forms"],
+                                                  float(np.max(np.abs(prices - ref))))
+    best = min((1, 2, 3), key=lambda v: out[IC_FORMS[v]])
+    chosen = best - 1 if out[IC_FORMS[best]] < 0.99 * out["auto"] else -1
+    return out, chosen
+
+
+def read_table(path):
+    rows = {}
+    if os.path.exists(path):
+        for line in open(path):
+            f = line.split()
+            if line.startswith("#") or len(f) < 3:
+                continue
+            rows[(int(f[0]), int(f[1]))] = [int(f[2]), int(f[3]) if len(f) > 3 else -1]
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, default=1024, help="options per shape")
+    ap.add_argument("--n-ic", type=int, default=768, help="options per shape for the launch-form timings")
     ap.add_argument("--reps", type=int, default=1)
+    ap.add_argument("--ic-only", action="store_true", help="keep the factorisation column, time the launch forms")
     ap.add_argument("--out", default=HERE, help="directory for the table and the record")
     args = ap.parse_args()
     table, record = os.path.join(args.out, os.path.basename(TABLE)), os.path.join(args.out, os.path.basename(RECORD))
     t0 = time.perf_counter()
-    rows, rec = [], []
+    old_rows = read_table(TABLE)
+    old_rec = json.load(open(RECORD)) if (args.ic_only and os.path.exists(RECORD)) else {}
+    rows, rec, ic_rec = [], [], []
     for shape in SHAPES:
-        dims, res, d = measure(shape, args.n, args.reps)
-        best = min(PATHS, key=lambda v: res[PATHS[v]]["ms_radau"])
-        gain = res["generic"]["ms_radau"] / res[PATHS[best]]["ms_radau"]
-        rows.append((dims[0], dims[1], best - 1))
-        rec.append({"n_s": shape[0], "n_v": shape[1], "n_sb": dims[0], "n_vb": dims[1], "chosen": PATHS[best],
-                    "speedup_vs_generic": round(gain, 4), "max_abs_dprice_between_paths": d, **res})
-        print(json.dumps(rec[-1]), flush=True)
+        if args.ic_only:
+            c = workloads.batch_columns(1, grid=shape)
+            _, _, info = api.price_batch(return_info=True, **dict(c, T=np.full(1, 0.02)))
+            dims = (info[0]["n_sb"], info[0]["n_vb"])
+            lu = old_rows.get(dims, [1, -1])[0]
+        else:
+            dims, res, d = measure(shape, args.n, args.reps)
+            best = min(PATHS, key=lambda v: res[PATHS[v]]["ms_radau"])
+            gain = res["generic"]["ms_radau"] / res[PATHS[best]]["ms_radau"]
+            lu = best - 1
+            rec.append({"n_s": shape[0], "n_v": shape[1], "n_sb": dims[0], "n_vb": dims[1], "chosen": PATHS[best],
+                        "speedup_vs_generic": round(gain, 4), "max_abs_dprice_between_paths": d, **res})
+            print(json.dumps(rec[-1]), flush=True)
+        ic, form = measure_ic(shape, args.n_ic, args.reps)
+        rows.append((dims[0], dims[1], lu, form))
+        ic_rec.append({"n_s": shape[0], "n_v": shape[1], "n_sb": dims[0], "n_vb": dims[1],
+                       "chosen": IC_FORMS[form + 1] if form >= 0 else "auto", "ms_ic": ic})
+        print(json.dumps(ic_rec[-1]), flush=True)
     os.makedirs(args.out, exist_ok=True)
     with open(table, "w") as f:
-        f.write("# n_sb n_vb variant (0 generic loops, 1 compiled k-steps); written by fpe_option_amd.autotune\n")
+        f.write("# n_sb n_vb variant admm_form (variant: 0 generic loops, 1 compiled k-steps; admm_form: k_admm launch "
+                "class 0..2, -1 by occupancy); written by fpe_option_amd.autotune\n")
         for r in rows:
-            f.write(f"{r[0]} {r[1]} {r[2]}\n")
+            f.write(f"{r[0]} {r[1]} {r[2]} {r[3]}\n")
+    out = dict(old_rec) if args.ic_only else {"options_per_shape": args.n, "reps": args.reps, "shapes": rec}
+    out["ic_forms"] = {"options_per_shape": args.n_ic, "maturity": 0.02, "reps": args.reps, "shapes": ic_rec}
+    out["seconds" if not args.ic_only else "seconds_ic"] = round(time.perf_counter() - t0, 1)
     with open(record, "w") as f:
-        json.dump({"options_per_shape": args.n, "reps": args.reps, "seconds": round(time.perf_counter() - t0, 1),
-                   "shapes": rec}, f, indent=1)
+        json.dump(out, f, indent=1)
     print("wrote", table, "and", record)
 
 

@@ -32,6 +32,7 @@ fpe::OptionSpec from_batch(const FpeBatchOption& b) {
     o.knot_width_s = b.knot_width_s, o.knot_width_v = b.knot_width_v;
     o.knot_center_v = b.knot_center_v;
     o.lu_variant = b.lu_variant, o.max_steps = b.max_steps, o.refine_max = b.refine_max;
+    o.ic_variant = b.ic_variant;
     return o;
 }
 

@@ -210,6 +210,7 @@ SolveSpec solve_for(const OptionSpec& o, int t) {
     s.knot_width_s = o.knot_width_s, s.knot_width_v = o.knot_width_v;
     s.knot_center_v = o.knot_center_v;
     s.lu_variant = o.lu_variant, s.max_steps = o.max_steps, s.refine_max = o.refine_max;
+    s.ic_variant = o.ic_variant;
     return s;
 }
 

@@ -26,6 +26,7 @@ struct OptionSpec {
     int lu_variant;        // factorisation code path: 0 tuned, 1 generic, 2 compiled k-steps
     int max_steps;         // Radau step limit (<= 0: 100000, the reference's)
     int refine_max;        // mixed precision: refinement rounds per stage solve (0: 10, < 0: none)
+    int ic_variant;        // initial-condition ADMM launch form: 0 tuned, 1..3 forced (FpeBatchOption)
 };
 
 // One Fokker-Planck solve on a (possibly barrier-truncated) domain.
@@ -38,7 +39,7 @@ struct SolveSpec {
     int complex_mode, precision, exact_constants;
     double knot_width_s, knot_width_v;
     double knot_center_v;
-    int lu_variant, max_steps, refine_max;
+    int lu_variant, max_steps, refine_max, ic_variant;
 };
 
 // Knots and quadrature of one solve.

@@ -83,11 +83,13 @@ struct DevPtrs {
 std::mutex g_mu;
 DeviceState g_dev;
 
-// Tuning table: measured factorisation code path per grid shape (n_sb, n_vb),
-// read once from tuning_gfx950.txt next to this library (written by
-// fpe_option_amd/autotune.py from GPU timings). Lines: "n_sb n_vb variant".
+// Tuning table: measured factorisation code path per grid shape (n_sb, n_vb)
+// and, optionally, the initial-condition ADMM launch form, read once from
+// tuning_gfx950.txt next to this library (written by fpe_option_amd/autotune.py
+// from GPU timings). Lines: "n_sb n_vb variant [admm_form]" (admm_form: the
+// k_admm launch class 0..2; absent or -1: chosen by occupancy).
 struct TuneRow {
-    int nsb, nvb, variant;
+    int nsb, nvb, variant, ic;
 };
 std::vector<TuneRow> g_tune;
 bool g_tune_loaded = false;
@@ -106,8 +108,12 @@ void load_tuning() {
     while (std::fgets(line, sizeof line, f)) {
         TuneRow r;
         if (line[0] == '#') continue;
-        if (std::sscanf(line, "%d %d %d", &r.nsb, &r.nvb, &r.variant) == 3 && (r.variant == 0 || r.variant == 1))
+        r.ic = -1;
+        const int got = std::sscanf(line, "%d %d %d %d", &r.nsb, &r.nvb, &r.variant, &r.ic);
+        if (got >= 3 && (r.variant == 0 || r.variant == 1)) {
+            if (got < 4 || r.ic < -1 || r.ic > 2) r.ic = -1;
             g_tune.push_back(r);
+        }
     }
     std::fclose(f);
 }
@@ -194,17 +200,34 @@ const AdmmKernel kAdmm4[2][4] = {
 // 2 when its work matrices fit the budget and it has at most 8 strips), the one
 // with the most resident workgroups per CU (g_admm_occ, from the runtime's
 // occupancy of each kernel at its budget) wins; ties go to the lower class.
+// want (0..2): a class asked for (forced by the caller, or measured best in the
+// tuning table), taken when the instance fits it.
 const long long kAdmmBudgets[3] = {ADMM_LDS_MAX, IC_LDS_HALF, IC_LDS_THIRD};
 int g_admm_occ[3][3][4];  // [class][MT - 1][TN - 1] workgroups per CU (class 0: MT up to 3, else 2)
-int admm_key(int ns, int nv) {
+bool admm_fits(int ns, int nv, int c) {
+    return c == 0 || (((ns + 15) >> 4) <= 8 && admm_lds_bytes(ns, nv, false) <= kAdmmBudgets[c]);
+}
+int admm_key(int ns, int nv, int want) {
     const int tn = admm_tn(nv);
-    int best = 0, best_occ = g_admm_occ[0][admm_mt(ns, 8) - 1][tn - 1];
-    if (((ns + 15) >> 4) <= 8)
+    int best = 0;
+    if (want >= 0 && want <= 2 && admm_fits(ns, nv, want)) {
+        best = want;
+    } else {
+        int best_occ = g_admm_occ[0][admm_mt(ns, 8) - 1][tn - 1];
         for (int c = 1; c < 3; ++c)
-            if (admm_lds_bytes(ns, nv, false) <= kAdmmBudgets[c] && g_admm_occ[c][admm_mt(ns, 4) - 1][tn - 1] > best_occ)
+            if (admm_fits(ns, nv, c) && g_admm_occ[c][admm_mt(ns, 4) - 1][tn - 1] > best_occ)
                 best = c, best_occ = g_admm_occ[c][admm_mt(ns, 4) - 1][tn - 1];
+    }
     const int mt = admm_mt(ns, best == 0 ? 8 : 4);
     return best * 16 + (mt - 1) * 4 + (tn - 1);
+}
+
+// The measured ADMM launch class of a grid shape (exact match), else -1.
+int tuned_admm_class(int nsb, int nvb) {
+    load_tuning();
+    for (const TuneRow& r : g_tune)
+        if (r.nsb == nsb && r.nvb == nvb) return r.ic;
+    return -1;
 }
 
 bool dev_init(std::string& err) {
@@ -436,7 +459,12 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
     for (int& k : ord_e0) k = s_reps[k];  // group order over instances (one per distinct s basis)
     ic_groups(n, [&](int s) { return eig_class(hi[s].nvb); },
               [&](int s) { return double(hi[s].nvb) * hi[s].nvb * hi[s].nvb; }, ord_e1, grp_e1);
-    ic_groups(n, [&](int s) { return admm_key(hi[s].nsb, hi[s].nvb); },
+    std::vector<int> admm_want(n);  // forced form (ic_variant 1..3), else the tuning table's, else -1
+    for (int s = 0; s < n; ++s) {
+        const int v = solves[s].ic_variant;
+        admm_want[s] = (v >= 1 && v <= 3) ? v - 1 : tuned_admm_class(hi[s].nsb, hi[s].nvb);
+    }
+    ic_groups(n, [&](int s) { return admm_key(hi[s].nsb, hi[s].nvb, admm_want[s]); },
               [&](int s) { return double(hi[s].nsb) * hi[s].nvb * (hi[s].nsb + hi[s].nvb); }, ord_a, grp_a);
     // offsets of the group orders after the Radau order
     const long long oo_e0 = n, oo_e1 = oo_e0 + (long long)ord_e0.size(), oo_a = oo_e1 + n;

@@ -166,7 +166,10 @@ typedef struct FpeBatchOption {
                            2 MFMA loops compiled for the block width's k-steps */
     int32_t max_steps;  /* Radau step limit; <= 0 -> 100000 (the reference's) */
     int32_t refine_max; /* FPE_PREC_MIXED refinement rounds per stage solve; 0 -> 10, < 0 -> none */
-    int32_t reserved;
+    int32_t ic_variant; /* initial-condition ADMM launch form: 0 tuned per grid (tuning_gfx950.txt, else the
+                           most resident workgroups per CU), 1 8-wave workgroups with up to 159 KB of LDS,
+                           2 / 3 4-wave workgroups in a half / third of a CU's LDS; a form the grid does not
+                           fit is replaced by 0's choice. Same results in every form up to reduction order. */
 } FpeBatchOption;
 
 /** Per-batch counters (summed over the solves of the batch). */
