@@ -42,3 +42,18 @@ def test_shared_s_grid_across_types_and_strikes():
     for i in range(n):
         p1, s1 = _single(cols, i)
         assert s1[0] == 0 and p1[0] == prices[i], (i, p1[0], prices[i])
+
+
+def test_initial_condition_launch_forms_agree():
+    # every ADMM launch form (FpeBatchOption.ic_variant 1..3, and 0: the library's choice)
+    # computes the same products; only the residual norms' partial sums are split differently
+    cols = workloads.batch_columns(16, seed=5)
+    cols["n_s"] = np.array([40, 80, 120, 60] * 4)
+    cols["n_v"] = np.array([20, 40, 30, 60] * 4)
+    cols["T"] = np.full(16, 0.05)
+    ref, status = api.price_batch(**cols, ic_variant=0)
+    assert np.all(status == 0)
+    for v in (1, 2, 3):
+        p, s = api.price_batch(**cols, ic_variant=v)
+        assert np.all(s == 0)
+        np.testing.assert_allclose(p, ref, rtol=1e-11, atol=1e-13, err_msg=f"ic_variant {v}")
