@@ -68,6 +68,70 @@ def test_reference_cpp_wrapper_compiles_against_this_header(tmp_path):
     assert r.returncode == 0, r.stderr
 
 
+@pytest.mark.skipif(not os.path.exists(REF_HPP), reason="the reference tree is not present")
+def test_reference_cpp_wrapper_runs_against_this_library(tmp_path, oracle):
+    # the reference's RAII class (cpp/include/fpe_compute.hpp:104-260), compiled against this header and linked to
+    # libfpe_engine.so: construction, the host-side calls, the exception on invalid parameters and the move
+    (tmp_path / "fpe_compute.hpp").symlink_to(REF_HPP)
+    src = tmp_path / "use.cpp"
+    src.write_text(r"""
+#include "fpe_compute.hpp"
+#include <cstdio>
+#include <stdexcept>
+#include <utility>
+static void put(const char* key, const std::vector<double>& v) {
+    std::printf("%s %zu", key, v.size());
+    for (double x : v) std::printf(" %.17g", x);
+    std::printf("\n");
+}
+int main() {
+    fpe::FpeCompute c(1.2, 0.05, 0.35, -0.4, 0.1, 0.6, 60.0, 0.1, 16, 16, 50.0, 2, 50, -1.0, 180.0);
+    const auto k = c.knots();
+    put("knots_s", k.s);
+    put("knots_v", k.v);
+    const auto g = c.grid_points();
+    put("grid_s", g.s);
+    put("grid_v", g.v);
+    put("grid_sw", g.s_weights);
+    put("grid_vw", g.v_weights);
+    bool threw = false;
+    try {
+        fpe::FpeCompute bad(-1.0, 0.05, 0.35, -0.4, 0.1, 0.6, 60.0, 0.1, 16, 16, 50.0, 2);
+    } catch (const std::runtime_error&) {
+        threw = true;
+    }
+    std::printf("invalid_throws %d\n", threw ? 1 : 0);
+    fpe::FpeCompute moved = std::move(c);
+    std::printf("moved_valid %d %d\n", moved.valid() ? 1 : 0, c.valid() ? 1 : 0);
+    return 0;
+}
+""")
+    root = os.path.dirname(HERE)
+    libdir = os.path.join(root, "fpe_option_amd")
+    exe = tmp_path / "use"
+    r = subprocess.run(["g++", "-std=c++17", "-O1", "-I", str(tmp_path), "-I", os.path.join(root, "include"), str(src),
+                        "-o", str(exe), "-L", libdir, "-lfpe_engine", f"-Wl,-rpath,{libdir}"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = {}
+    for line in r.stdout.splitlines():
+        parts = line.split()
+        out[parts[0]] = parts[1:]
+    P = oracle.Pipe(**DOC)
+    s, v = P.knots()
+    np.testing.assert_array_equal(_vec(out, "knots_s"), s)
+    np.testing.assert_array_equal(_vec(out, "knots_v"), v)
+    g = P.grid()
+    np.testing.assert_array_equal(_vec(out, "grid_s"), g["s"])
+    np.testing.assert_array_equal(_vec(out, "grid_v"), g["v"])
+    np.testing.assert_array_equal(_vec(out, "grid_sw"), g["sw"])
+    np.testing.assert_array_equal(_vec(out, "grid_vw"), g["vw"])
+    assert out["invalid_throws"] == ["1"]
+    assert out["moved_valid"] == ["1", "0"]
+
+
 @pytest.mark.gpu
 def test_c_caller_pricing_calls(engine, oracle):
     out = _run("gpu")
