@@ -66,17 +66,19 @@ def in_box(P):
 
 
 def _options(P, strike=None, otype=None):
+    """FpeBatchOption array on the surrogate's grid, written through a numpy view."""
+    P = np.asarray(P, dtype=np.float64)
     n = len(P)
     arr = L.batch_options(n)
-    for i in range(n):
-        o = arr[i]
+    view = np.ctypeslib.as_array(arr) if n else None
+    if n:
         for k, v in GRID.items():
-            setattr(o, k, v)
+            view[k] = v
         for j, name in enumerate(PARAMS):
-            setattr(o, name, float(P[i, j]))
-        o.strike = float(strike[i]) if strike is not None else 60.0
-        o.option_type = int(otype[i]) if otype is not None else 8
-        o.barrier = 0.0
+            view[name] = P[:, j]
+        view["strike"] = np.asarray(strike, dtype=np.float64) if strike is not None else 60.0
+        view["option_type"] = np.asarray(otype, dtype=np.int32) if otype is not None else 8
+        view["barrier"] = 0.0
     return arr
 
 
