@@ -65,6 +65,10 @@ struct DeviceState {
     int n_cu = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[8];
+    // streams beside `stream` for independent launch groups (default priority),
+    // and the events that fork them from it and join them back
+    hipStream_t aux[3] = {};
+    hipEvent_t fork_ev = nullptr, join_ev[3] = {};
     unsigned char* scratch = nullptr;  // device address of `scratch` on this device
     DBuf inst, out, kn, qd, bv, bc, fc, st, vc, eg, hs, wm, ow, ws, wi, req, part, queue, order, pdf, kron;
 };
@@ -261,6 +265,9 @@ bool dev_init(std::string& err) {
                                int(EIG_LDS_S)));
     HIP_OK(hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking));
     for (auto& e : g_dev.ev) HIP_OK(hipEventCreate(&e));
+    for (auto& a : g_dev.aux) HIP_OK(hipStreamCreateWithFlags(&a, hipStreamNonBlocking));
+    HIP_OK(hipEventCreateWithFlags(&g_dev.fork_ev, hipEventDisableTiming));
+    for (auto& e : g_dev.join_ev) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     // once per device, with the device current: the scratch array's device address
     HIP_OK(hipGetSymbolAddress(reinterpret_cast<void**>(&g_dev.scratch), HIP_SYMBOL(scratch)));
     if (!g_dev.scratch) {
@@ -588,22 +595,72 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord(G.ev[1], st));
     if (!given) {
+        // Two stages of mutually independent launches, each spread over the batch
+        // stream and the auxiliary ones (fork, launches, join), so that one launch's
+        // last workgroups overlap the next launch's first: (1) the right-hand sides
+        // and the s and v eigenproblem groups (disjoint outputs in VC and EG); (2) the
+        // ADMM groups (disjoint instances; EG and the stencils read only). A group goes
+        // to the stream with the least estimated work so far, largest group first.
+        constexpr int NS = 1 + int(sizeof(G.aux) / sizeof(G.aux[0]));
+        const hipStream_t sts[NS] = {st, G.aux[0], G.aux[1], G.aux[2]};
+        struct Job {
+            double cost;
+            int kind, g;  // kind 0: s eigenproblems, 1: v eigenproblems, 2: ADMM; g: group index
+        };
+        auto group_cost = [&](const std::vector<int>& ord, const IcGroup& g, auto cost) {
+            double c = 0.0;
+            for (int i = g.start; i < g.start + g.count; ++i) c += cost(ord[i]);
+            return c;
+        };
+        auto launch = [&](const Job& j, hipStream_t s_) {
+            if (j.kind < 2) {
+                const IcGroup& g = (j.kind == 0 ? grp_e0 : grp_e1)[j.g];
+                hipLaunchKernelGGL(k_eig, dim3(g.count), dim3(ENT), kEigBudgets[g.key], s_, dI,
+                                   D.order + (j.kind == 0 ? oo_e0 : oo_e1) + g.start, j.kind, kEigBudgets[g.key],
+                                   D.fc, D.eg);
+            } else {
+                const IcGroup& g = grp_a[j.g];
+                const int cls = g.key / 16, mt = (g.key / 4) % 4 + 1, tn = g.key % 4 + 1;
+                const long long lds = kAdmmBudgets[cls];
+                const AdmmKernel k = cls == 0 ? kAdmm8[mt - 1][tn - 1] : kAdmm4[mt - 1][tn - 1];
+                hipLaunchKernelGGL(k, dim3(g.count), dim3((cls == 0 ? 8 : 4) * 64), lds, s_, dI,
+                                   D.order + oo_a + g.start, D.st, D.eg, D.vc, D.out, lds);
+            }
+        };
+        auto stage = [&](std::vector<Job> jobs) -> bool {
+            std::stable_sort(jobs.begin(), jobs.end(), [](const Job& x, const Job& y) { return x.cost > y.cost; });
+            HIP_OK(hipEventRecord(G.fork_ev, st));
+            for (int i = 1; i < NS; ++i) HIP_OK(hipStreamWaitEvent(sts[i], G.fork_ev, 0));
+            double load[NS] = {};
+            for (const Job& j : jobs) {
+                int best = 0;
+                for (int i = 1; i < NS; ++i)
+                    if (load[i] < load[best]) best = i;
+                launch(j, sts[best]);
+                load[best] += j.cost;
+            }
+            HIP_OK(hipGetLastError());
+            for (int i = 1; i < NS; ++i) {
+                HIP_OK(hipEventRecord(G.join_ev[i - 1], sts[i]));
+                HIP_OK(hipStreamWaitEvent(st, G.join_ev[i - 1], 0));
+            }
+            return true;
+        };
         hipLaunchKernelGGL(k_ic_rhs, dim3(n), dim3(NT), 0, st, dI, D.order, D.qd,
                            D.bv, D.bc, D.vc);
-        for (const IcGroup& g : grp_e0)
-            hipLaunchKernelGGL(k_eig, dim3(g.count), dim3(ENT), kEigBudgets[g.key], st, dI, D.order + oo_e0 + g.start,
-                               0, kEigBudgets[g.key], D.fc, D.eg);
-        for (const IcGroup& g : grp_e1)
-            hipLaunchKernelGGL(k_eig, dim3(g.count), dim3(ENT), kEigBudgets[g.key], st, dI, D.order + oo_e1 + g.start,
-                               1, kEigBudgets[g.key], D.fc, D.eg);
-        for (const IcGroup& g : grp_a) {
-            const int cls = g.key / 16, mt = (g.key / 4) % 4 + 1, tn = g.key % 4 + 1;
-            const long long lds = kAdmmBudgets[cls];
-            const AdmmKernel k = cls == 0 ? kAdmm8[mt - 1][tn - 1] : kAdmm4[mt - 1][tn - 1];
-            hipLaunchKernelGGL(k, dim3(g.count), dim3((cls == 0 ? 8 : 4) * 64), lds, st, dI,
-                               D.order + oo_a + g.start, D.st, D.eg, D.vc, D.out, lds);
-        }
-        HIP_OK(hipGetLastError());
+        std::vector<Job> jobs;
+        auto cube = [](double x) { return x * x * x; };
+        for (int i = 0; i < int(grp_e0.size()); ++i)
+            jobs.push_back({group_cost(ord_e0, grp_e0[i], [&](int s) { return cube(hi[s].nsb); }), 0, i});
+        for (int i = 0; i < int(grp_e1.size()); ++i)
+            jobs.push_back({group_cost(ord_e1, grp_e1[i], [&](int s) { return cube(hi[s].nvb); }), 1, i});
+        if (!stage(jobs)) return false;
+        jobs.clear();
+        for (int i = 0; i < int(grp_a.size()); ++i)
+            jobs.push_back({group_cost(ord_a, grp_a[i],
+                                       [&](int s) { return double(hi[s].nsb) * hi[s].nvb * (hi[s].nsb + hi[s].nvb); }),
+                            2, i});
+        if (!stage(jobs)) return false;
     }
     HIP_OK(hipEventRecord(G.ev[2], st));
     if (solved && !given) {
