@@ -165,7 +165,11 @@ __device__ __forceinline__ double band(const double* F, int nb, int f, int i, in
 // K = S1(x)Nv + S2(x)V2 + Ns(x)(k5 V4 + k6 V5) + S4(x)V2 + k8 Ns(x)V6 + (S3(x)V3)^T + S3(x)V3
 // into 49-point stencils, scaled as D^-1 M D^-1 with D = sqrt(diag M).
 // Stencil entry e = (J-I+3)*7 + (j-i+3) is stored plane-major: ST[o + e*N + r]
-// (M planes 0..48, K planes 49..97).  grid: (n_inst, row chunks).
+// (M planes 0..48, K planes 49..97). A workgroup takes 256 consecutive rows at a
+// time; D^-1 of those rows and of the 3 n_vb + 3 rows on either side (every
+// row a stencil of the block reaches) is computed once into LDS.
+// grid: (n_inst, row chunks).
+constexpr int STENCIL_HALO = 3 * 64 + 3;  // 3 n_vb + 3 rows, n_vb <= 64
 __global__ void __launch_bounds__(NT) k_stencil(const Inst* __restrict__ inst, const double* __restrict__ FC,
                                                 double* __restrict__ ST, double* __restrict__ VC) {
     const Inst& in = inst[blockIdx.x];
@@ -178,36 +182,68 @@ __global__ void __launch_bounds__(NT) k_stencil(const Inst* __restrict__ inst, c
     double* Ks = Ms + 49LL * N;
     double* D = VC + in.o_vec + (long long)V_D * N;
     double* Di = VC + in.o_vec + (long long)V_DINV * N;
-    for (int r = blockIdx.y * NT + threadIdx.x; r < N; r += gridDim.y * NT) {
+    __shared__ double dinv[NT + 2 * STENCIL_HALO];
+    const int H = 3 * nv + 3;
+    for (int r0 = blockIdx.y * NT; r0 < N; r0 += gridDim.y * NT) {
+        __syncthreads();
+        for (int k = threadIdx.x; k < NT + 2 * H; k += NT) {
+            const int rr = r0 - H + k;
+            if (rr >= 0 && rr < N) {
+                const int J = rr / nv, jj = rr - J * nv;
+                const double m = band(Fs, ns, 0, J, J) * band(Fv, nv, 0, jj, jj);
+                dinv[k] = m > 1e-30 ? 1.0 / sqrt(m) : 1.0;
+            }
+        }
+        __syncthreads();
+        const int r = r0 + threadIdx.x;
+        if (r >= N) continue;
         const int I = r / nv, i = r - I * nv;
-        auto dinv = [&](int J, int jj) {
-            double m = band(Fs, ns, 0, J, J) * band(Fv, nv, 0, jj, jj);
-            return m > 1e-30 ? 1.0 / sqrt(m) : 1.0;
-        };
         const double mrr = band(Fs, ns, 0, I, I) * band(Fv, nv, 0, i, i);
         const double dr = mrr > 1e-30 ? sqrt(mrr) : 1.0;
         const double dir = mrr > 1e-30 ? 1.0 / dr : 1.0;
         D[r] = dr;
         Di[r] = dir;
+        const double* dn = dinv + (threadIdx.x + H);  // D^-1 of row r + o at dn[o], |o| <= H
+        // the row's v-factor entries (i, i + dj) and the transposed V3 (i + dj, i), loaded
+        // once and used by all seven block offsets dJ (each s-factor entry then by seven dj)
+        double v0[7], v1[7], v2[7], v2t[7], v3[7], v4[7], v5[7];
+        bool vin[7];
+#pragma unroll
+        for (int q = 0; q < 7; ++q) {
+            const int jj = i + q - 3;
+            vin[q] = jj >= 0 && jj < nv;
+            const int jc = vin[q] ? jj : i;
+            v0[q] = band(Fv, nv, 0, i, jc);
+            v1[q] = band(Fv, nv, 1, i, jc);
+            v2[q] = band(Fv, nv, 2, i, jc);
+            v2t[q] = band(Fv, nv, 2, jc, i);
+            v3[q] = band(Fv, nv, 3, i, jc);
+            v4[q] = band(Fv, nv, 4, i, jc);
+            v5[q] = band(Fv, nv, 5, i, jc);
+        }
         for (int dJ = -3; dJ <= 3; ++dJ) {
             const int J = I + dJ;
-            for (int dj = -3; dj <= 3; ++dj) {
-                const int jj = i + dj;
-                const int e = (dJ + 3) * 7 + (dj + 3);
+            const bool Jin = J >= 0 && J < ns;
+            const int Jc = Jin ? J : I;
+            const double s0 = band(Fs, ns, 0, I, Jc), s1 = band(Fs, ns, 1, I, Jc), s2 = band(Fs, ns, 2, I, Jc);
+            const double s3 = band(Fs, ns, 3, I, Jc), s3t = band(Fs, ns, 3, Jc, I), s4 = band(Fs, ns, 4, I, Jc);
+#pragma unroll
+            for (int q = 0; q < 7; ++q) {
+                const int e = (dJ + 3) * 7 + q;
                 double mv = 0.0, kv = 0.0;
-                if (J >= 0 && J < ns && jj >= 0 && jj < nv) {
-                    const double ns_ = band(Fs, ns, 0, I, J);
-                    const double nv_ = band(Fv, nv, 0, i, jj);
+                if (Jin && vin[q]) {
+                    const double ns_ = s0;
+                    const double nv_ = v0[q];
                     mv = ns_ * nv_;
-                    const double a = band(Fs, ns, 1, I, J) * nv_;
-                    const double b = band(Fs, ns, 2, I, J) * band(Fv, nv, 1, i, jj);
-                    const double c = ns_ * (k5 * band(Fv, nv, 3, i, jj) + k6 * band(Fv, nv, 4, i, jj));
-                    const double d = band(Fs, ns, 4, I, J) * band(Fv, nv, 1, i, jj);
-                    const double f = (k8 * ns_) * band(Fv, nv, 5, i, jj);
-                    const double g = band(Fs, ns, 3, J, I) * band(Fv, nv, 2, jj, i);
-                    const double h = band(Fs, ns, 3, I, J) * band(Fv, nv, 2, i, jj);
+                    const double a = s1 * nv_;
+                    const double b = s2 * v1[q];
+                    const double c = ns_ * (k5 * v3[q] + k6 * v4[q]);
+                    const double d = s4 * v1[q];
+                    const double f = (k8 * ns_) * v5[q];
+                    const double g = s3t * v2t[q];
+                    const double h = s3 * v2[q];
                     kv = (((((a + b) + c) + d) + f) + g) + h;
-                    const double sc = dir * dinv(J, jj);
+                    const double sc = dir * dn[dJ * nv + q - 3];
                     mv = sc * mv;
                     kv = sc * kv;
                 }
