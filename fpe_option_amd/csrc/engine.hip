@@ -446,15 +446,21 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
     L.istride = ((2 * max_nm) + 63) & ~63LL;
     L.has_complex = any_direct;
     int n_slots = std::min(n, std::max(1, RADAU_WG_PER_CU * g_dev.n_cu));
-    // Radau work queue order: largest estimated factorisation cost (n_sb n_vb^3) first,
-    // so the slots finish together instead of waiting on a late large instance.
+    // Radau work queue order: largest estimated cost first, so the slots finish
+    // together instead of waiting on a late large instance. Cost per instance
+    // n_sb n_vb^2 (n_vb + k): the factorisations grow as n_sb n_vb^3, the block
+    // solves and operator products as n_sb n_vb^2, and k weighs the second by the
+    // mode's measured share of the solver's time at n_vb = 40 (GMRES mode: solves and
+    // products three times the factorisation time; mixed precision, with its refined
+    // solves: 0.6 times; direct FP64: 0.3 times).
+    std::vector<double> rcost(n);
+    for (int s = 0; s < n; ++s) {
+        const double k = hi[s].prec == 1 ? 24.0 : hi[s].cmode == 1 ? 11.0 : 120.0;
+        rcost[s] = double(hi[s].nsb) * hi[s].nvb * hi[s].nvb * (hi[s].nvb + k);
+    }
     std::vector<int> order(n);
     for (int s = 0; s < n; ++s) order[s] = s;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-        const double ca = double(hi[a].nsb) * hi[a].nvb * hi[a].nvb * hi[a].nvb;
-        const double cb = double(hi[b].nsb) * hi[b].nvb * hi[b].nvb * hi[b].nvb;
-        return ca > cb;
-    });
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return rcost[a] > rcost[b]; });
 
     // initial-condition launch groups: the s and v eigenproblems (A and Qt in
     // LDS when the class budget holds both), then ADMM
