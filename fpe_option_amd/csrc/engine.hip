@@ -652,8 +652,8 @@ bool run_batch(const std::vector<SolveSpec>& solves, const std::vector<Domain>& 
             }
             return true;
         };
-        hipLaunchKernelGGL(k_ic_rhs, dim3(n), dim3(NT), 0, st, dI, D.order, D.qd,
-                           D.bv, D.bc, D.vc);
+        static_assert(MAX_SB <= 2 * NT, "k_ic_rhs sums two k per thread");
+        hipLaunchKernelGGL(k_ic_rhs, dim3(n, max_m), dim3(NT), 0, st, dI, D.order, D.qd, D.bv, D.bc, D.vc);
         std::vector<Job> jobs;
         auto cube = [](double x) { return x * x * x; };
         for (int i = 0; i < int(grp_e0.size()); ++i)

@@ -33,12 +33,21 @@ struct EigView {
 };
 
 // Right-hand side b~ = D (Bs (x) Bv)^T W delta and the basis integrals
-// m = (Bs (x) Bv)^T W 1 (separable).  grid: n_inst.
+// m = (Bs (x) Bv)^T W 1 (separable):
+//     b[k][l] = sum_p Bs[p][k] T_l(p),  T_l(p) = sum_q Bv[q][l] (w_p w_q delta(p, q)),
+// p ascending over the s points in the support of k, q ascending over the v
+// points in the support of l. One workgroup per (instance, l): the threads first
+// compute T_l(p) for a block of IC_PBLK s points into LDS (one p each), then one
+// thread per k adds its points of the block in ascending order.
+// grid: (n_inst, max n_vb).
+constexpr int IC_PBLK = 2048;  // s points per LDS block; thread t sums k = t and t + NT (n_sb <= 2 NT)
 __global__ void __launch_bounds__(NT) k_ic_rhs(const Inst* __restrict__ inst, const int* __restrict__ order, const double* __restrict__ QD,
                                                const double* __restrict__ BV, const int* __restrict__ BC,
                                                double* __restrict__ VC) {
     const Inst& in = inst[order[blockIdx.x]];  // largest instances first
     const int ns = in.nsb, nv = in.nvb, N = ns * nv, nqs = in.nqs, nqv = in.nqv;
+    const int l = blockIdx.y;
+    if (l >= nv) return;  // workgroup-uniform
     const double* sw = QD + in.o_sq + nqs;
     const double* sph = QD + in.o_sq + 2 * nqs;
     const double* vx = QD + in.o_vq;
@@ -50,51 +59,73 @@ __global__ void __launch_bounds__(NT) k_ic_rhs(const Inst* __restrict__ inst, co
     double* b = VC + in.o_vec + (long long)V_BT * N;
     double* mw = VC + in.o_vec + (long long)V_MW * N;
     const double* Di = VC + in.o_vec + (long long)V_DINV * N;
-    __shared__ double om_s[320], om_v[320];
-    for (int r = threadIdx.x; r < N; r += NT) b[r] = 0.0;
-    // omega_s[k] = sum_p Bs[p,k] sw_p ; omega_v[l] = sum_q Bv[q,l] vw_q
+    __shared__ double Tl[IC_PBLK];
+    __shared__ double om_v;
+    const int q0 = lower_c0(cv, nqv, l - 3), q1 = lower_c0(cv, nqv, l + 1);
+    // omega_v[l] = sum_q Bv[q,l] vw_q ; omega_s[k] = sum_p Bs[p,k] sw_p
+    if (threadIdx.x == 0) {
+        double a = 0.0;
+        for (int q = q0; q < q1; ++q) a += Tv[(long long)q * 8 + (l - cv[q])] * vw[q];
+        om_v = a;
+    }
+    __syncthreads();
     for (int k = threadIdx.x; k < ns; k += NT) {
         double a = 0.0;
         const int p0 = lower_c0(cs, nqs, k - 3), p1 = lower_c0(cs, nqs, k + 1);
         for (int p = p0; p < p1; ++p) a += Ts[(long long)p * 8 + (k - cs[p])] * sw[p];
-        om_s[k] = a;
+        mw[(long long)k * nv + l] = a * om_v;
     }
-    for (int l = threadIdx.x; l < nv; l += NT) {
-        double a = 0.0;
-        const int q0 = lower_c0(cv, nqv, l - 3), q1 = lower_c0(cv, nqv, l + 1);
-        for (int q = q0; q < q1; ++q) a += Tv[(long long)q * 8 + (l - cv[q])] * vw[q];
-        om_v[l] = a;
-    }
-    __syncthreads();
-    for (int r = threadIdx.x; r < N; r += NT) mw[r] = om_s[r / nv] * om_v[r % nv];
     // truncated product Gaussian (rho = 0), initial_cond.mojo:25-67
     const double sig_s = in.ic_sig_s, sig_v = in.ic_sig_v;
     const double norm = 1.0 / (2.0 * M_PI * sig_s * sig_v * 1.0);
-    __syncthreads();
-    for (int p = 0; p < nqs; ++p) {
-        const double ds = (sph[p] - in.S0) / sig_s, ds2 = ds * ds;
-        if (ds2 > 50.0) continue;  // uniform across the workgroup
-        for (int l = threadIdx.x; l < nv; l += NT) {
-            const int q0 = lower_c0(cv, nqv, l - 3), q1 = lower_c0(cv, nqv, l + 1);
+    int kp0[2], kp1[2];
+    double acc[2] = {0.0, 0.0};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int k = threadIdx.x + h * NT;
+        kp0[h] = k < ns ? lower_c0(cs, nqs, k - 3) : 0;
+        kp1[h] = k < ns ? lower_c0(cs, nqs, k + 1) : 0;
+    }
+    for (int P0 = 0; P0 < nqs; P0 += IC_PBLK) {
+        const int P1 = nqs - P0 < IC_PBLK ? nqs : P0 + IC_PBLK;
+        __syncthreads();
+        for (int p = P0 + threadIdx.x; p < P1; p += NT) {
+            const double ds = (sph[p] - in.S0) / sig_s, ds2 = ds * ds;
             double T = 0.0;
-            for (int q = q0; q < q1; ++q) {
-                const double dv = (vx[q] - in.V0) / sig_v;
-                const double z = ds2 - 2.0 * 0.0 * ds * dv + dv * dv;
-                double dl = 0.0;
-                if (!(z * 0.5 > 50.0)) {
-                    dl = norm * exp(-z * 0.5);
-                    if (dl < 1e-6) dl = 0.0;
+            if (!(ds2 > 50.0)) {
+                for (int q = q0; q < q1; ++q) {
+                    const double dv = (vx[q] - in.V0) / sig_v;
+                    const double z = ds2 - 2.0 * 0.0 * ds * dv + dv * dv;
+                    double dl = 0.0;
+                    if (!(z * 0.5 > 50.0)) {
+                        dl = norm * exp(-z * 0.5);
+                        if (dl < 1e-6) dl = 0.0;
+                    }
+                    T += Tv[(long long)q * 8 + (l - cv[q])] * (sw[p] * vw[q] * dl);
                 }
-                T += Tv[(long long)q * 8 + (l - cv[q])] * (sw[p] * vw[q] * dl);
             }
-            for (int t = 0; t < 4; ++t) {
-                const int k = cs[p] + t;
-                if (k < ns) b[(long long)k * nv + l] += Ts[(long long)p * 8 + t] * T;
+            Tl[p - P0] = T;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int k = threadIdx.x + h * NT;
+            const int pa = kp0[h] > P0 ? kp0[h] : P0, pb = kp1[h] < P1 ? kp1[h] : P1;
+            for (int p = pa; p < pb; ++p) {
+                const double ds = (sph[p] - in.S0) / sig_s, ds2 = ds * ds;
+                if (ds2 > 50.0) continue;  // the points the sum skips
+                acc[h] += Ts[(long long)p * 8 + (k - cs[p])] * Tl[p - P0];
             }
         }
     }
-    __syncthreads();
-    for (int r = threadIdx.x; r < N; r += NT) b[r] = Di[r] * b[r];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int k = threadIdx.x + h * NT;
+        if (k < ns) {
+            const long long r = (long long)k * nv + l;
+            b[r] = Di[r] * acc[h];
+        }
+    }
 }
 
 // Jacobi eigendecomposition of As (axis 0) or Av (axis 1) with parallel
